@@ -1,0 +1,194 @@
+"""DF-JK Fock-build benchmark on one MI355X.
+
+Workload (one "step" = one complete J + K build at fixed density, inputs
+resident in HBM): all-trans C20H42, def2-SVP orbital basis (490 AO; the
+offline data has no carbon cc-pVDZ), auxiliary cc-pVDZ-JKFIT for H and the
+oxygen cc-pVDZ-JKFIT shells scaled by (6/8)^2 for C (2366 functions), 81
+occupied orbitals drawn N(0,1) from a seed and Lowdin-orthonormalised in the
+AO overlap.  The same plan is timed in FP64, FP32 and BF16X3 (split bfloat16,
+FP32 accumulation); `value` is the BF16X3 rate, the FP64 and FP32 rates and the
+max-abs J/K deviations from FP64 are reported beside it.  A second workload,
+(H2O)4 cc-pVDZ/cc-pVDZ-JKFIT, gives SCF seconds per molecule in FP64 and in
+mixed precision (BF16X3 iterations + FP64 refinement).
+
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu-baseline]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "DF-JK Fock builds/s & SCF s/molecule, mixed vs FP64, 1 GPU, ≤1 µEh error"
+HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
+
+
+def _alkane_plan(precision="fp64"):
+    from vibeqc_amd import AOBasis, FockPlan, shells_for
+    from vibeqc_amd.basis import cc_pvdz_aux_for_carbon
+    from vibeqc_amd.geometry import alkane
+
+    atoms = alkane(20)
+    orb = AOBasis(atoms, shells_for("def2-svp", atoms))
+    aux = AOBasis(atoms, shells_for(cc_pvdz_aux_for_carbon(), atoms))
+    t0 = time.perf_counter()
+    plan = FockPlan(orb, auxiliary=aux, precision=precision)
+    return plan, time.perf_counter() - t0
+
+
+def _cpu_baseline(n, naux, nocc, seed=7):
+    """The FP64 oracle's occupied-factor J/K (numpy/BLAS) on a bounded sample:
+    synthetic B of shape (n, n, naux/4), scaled x4 (the build is linear in naux)."""
+    from oracle import oracle as O
+
+    try:
+        from threadpoolctl import threadpool_info
+
+        threads = max((p.get("num_threads", 1) for p in threadpool_info() if p.get("user_api") == "blas"), default=1)
+    except Exception:
+        threads = os.cpu_count() or 1
+    sub = max(1, naux // 4)
+    rng = np.random.default_rng(seed)
+    B = rng.standard_normal((n, n, sub))
+    B = 0.5 * (B + B.transpose(1, 0, 2))
+    C = rng.standard_normal((n, nocc))
+    O.jk_occupied(B, C)  # prime
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        O.jk_occupied(B, C)
+        runs.append(time.perf_counter() - t0)
+    per_build = float(np.median(runs)) * (naux / sub)
+    return {
+        "value": 1.0 / per_build,
+        "unit": "Fock builds/s",
+        "cores": int(threads),
+        "kind": "port",
+        "sample": f"oracle jk_occupied (numpy FP64) on synthetic B {n}x{n}x{sub} (naux/4), C {n}x{nocc}; median of 3 after a prime, scaled x{naux / sub:.2f} to naux={naux}",
+    }
+
+
+def _scf_timing(precision):
+    from vibeqc_amd import AOBasis, FockPlan, shells_for
+    from vibeqc_amd.geometry import water_cluster
+
+    atoms = water_cluster(4, 6.0, seed=0)
+    orb = AOBasis(atoms, shells_for("cc-pvdz", atoms))
+    aux = AOBasis(atoms, shells_for("cc-pvdz-jkfit", atoms))
+    t0 = time.perf_counter()
+    with FockPlan(orb, auxiliary=aux, precision=precision) as plan:
+        t1 = time.perf_counter()
+        r = plan.solve(energy_tolerance=1e-10, density_tolerance=1e-8)
+        t2 = time.perf_counter()
+    return {"energy": r.energy, "iterations": r.iterations, "setup_s": t1 - t0, "scf_s": t2 - t1, "total_s": t2 - t0}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            "bench.py measures one MI355X: this engine has no multi-GPU path in this build "
+            "(sharding over the auxiliary index is described in DESIGN.md). Run with --gpus 1.",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+
+    from vibeqc_amd.geometry import lowdin_occupied
+
+    plan, setup_s = _alkane_plan()
+    shape = plan.shape()
+    n, naux, npair = shape["nbf"], shape["naux"], shape["npair"]
+    nocc = 81
+    S, _ = plan.one_electron()
+    C = lowdin_occupied(S, nocc, seed=1)
+    j64, k64 = plan.evaluate_occupied(C)
+
+    results = {}
+    for precision in ("fp64", "fp32", "bf16x3"):
+        plan.set_precision(precision)
+        plan.benchmark_occupied(C, repeat=max(1, args.warmup))
+        t0 = time.perf_counter()
+        ms = plan.benchmark_occupied(C, repeat=args.steps)
+        wall = (time.perf_counter() - t0) / args.steps
+        j, k = plan.evaluate_occupied(C)
+        results[precision] = {
+            "builds_per_s": 1.0 / wall,
+            "ms_per_build_wall": wall * 1e3,
+            "phase_ms": ms,
+            "max_abs_err_J": float(np.abs(j - j64).max()),
+            "max_abs_err_K": float(np.abs(k - k64).max()),
+        }
+        print(f"[bench] {precision}: {1e3 * wall:.3f} ms/build, phases {ms}", file=sys.stderr, flush=True)
+
+    head = results["bf16x3"]
+    ph = head["phase_ms"]
+    gemm_flops = 2.0 * n * n * naux * nocc
+    # Dominant kernel of the headline build and its algorithmic work.
+    phases = {
+        "exchange_u": (ph["exchange_u_ms"], "mfma", gemm_flops / (ph["exchange_u_ms"] * 1e-3) / 1e12, PEAK_TFLOPS["bf16x3"], "TFLOP/s"),
+        "exchange_syrk": (ph["exchange_syrk_ms"], "mfma", gemm_flops / (ph["exchange_syrk_ms"] * 1e-3) / 1e12, PEAK_TFLOPS["bf16x3"], "TFLOP/s"),
+        "coulomb": (ph["coulomb_ms"], "hbm", 2.0 * npair * naux * 4 / (ph["coulomb_ms"] * 1e-3) / 1e9, HBM_PEAK_GBS, "GB/s"),
+    }
+    name, (t_ms, bound, achieved, peak, unit) = max(phases.items(), key=lambda kv: kv[1][0])
+    roofline = {
+        "kernel": name,
+        "bound": bound,
+        "achieved": achieved,
+        "peak": peak,
+        "unit": unit,
+        "frac": achieved / peak,
+        "traffic": None,
+    }
+    cpu = None if args.no_cpu_baseline else _cpu_baseline(n, naux, nocc)
+    scf64 = _scf_timing("fp64")
+    scfmx = _scf_timing("bf16x3")
+    plan.close()
+    line = {
+        "metric": METRIC,
+        "value": head["builds_per_s"],
+        "unit": "Fock builds/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_build_wall"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3",
+        "data": "synthetic",
+        "config": {
+            "workload": "C20H42 def2-SVP (490 AO) / jkfit-style aux (2366), DF-JK Fock build at fixed density, 81 occupied",
+            "nbf": n,
+            "naux": naux,
+            "nocc": nocc,
+            "parallelism": "single-gpu",
+            "setup_s": setup_s,
+        },
+        "fp64_builds_per_s": results["fp64"]["builds_per_s"],
+        "fp32_builds_per_s": results["fp32"]["builds_per_s"],
+        "mixed_speedup_vs_fp64": head["builds_per_s"] / results["fp64"]["builds_per_s"],
+        "precision_results": results,
+        "scf_water4": {"fp64": scf64, "mixed": scfmx, "energy_diff_mixed_minus_fp64": scfmx["energy"] - scf64["energy"]},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,73 @@
+"""Extract the reference's committed golden vectors into small local fixtures.
+
+Reads (as data only; nothing is imported or run from the reference):
+  * tests/reference_data/posthf/{h2,lih,water,f_heh}.{json,npz} -- PySCF 2.14 /
+    libcint DF tensors, DF Fock matrices and DF-RHF energies;
+  * benchmarks/results/issue409-packed-values/references/
+    {water,water-tetramer,water-hexadecamer-2s4}-def2-svp-spherical-reference.json
+    -- GPU4PySCF 1.8.1 cc-pVDZ / cc-pVDZ-JKFIT DF-RHF energies, forces and
+    geometries (the file names say def2-svp; the content is cc-pVDZ).
+The libcint scalar known answers of tests/native/test_density_fitting.cpp:468-499
+are copied into libcint_known_answers.json by hand (they are literals in that
+test, not a data file).
+
+Usage: python tests/golden/make_fixtures.py /root/reference
+"""
+
+from __future__ import annotations
+
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+
+POSTHF_KEYS = ("raw_three_center", "metric", "df_S", "df_h", "df_F", "df_C", "df_occ", "df_eps")
+
+
+def main(reference: str) -> None:
+    root = Path(reference)
+    posthf = root / "tests/reference_data/posthf"
+    for name in ("h2", "lih", "water", "f_heh"):
+        meta = json.loads((posthf / f"{name}.json").read_text())
+        arrays = np.load(posthf / f"{name}.npz", allow_pickle=False)
+        np.savez_compressed(HERE / f"posthf_{name}.npz", **{k: arrays[k] for k in POSTHF_KEYS})
+        record = {
+            "inputs": meta["inputs"],
+            "auxiliary_shells": meta["auxiliary_shells"],
+            "df_hf_energy": meta["records"]["df"]["hf_energy"],
+            "settings": meta["settings"],
+            "versions": {"pyscf": meta["versions"]["pyscf"]},
+        }
+        (HERE / f"posthf_{name}.json").write_text(json.dumps(record, indent=1) + "\n")
+
+    refs = root / "benchmarks/results/issue409-packed-values/references"
+    for stem, out in (
+        ("water-def2-svp-spherical-reference", "ccpvdz_water"),
+        ("water-tetramer-def2-svp-spherical-reference", "ccpvdz_water4"),
+        ("water-hexadecamer-2s4-def2-svp-spherical-reference", "ccpvdz_water16"),
+    ):
+        meta = json.loads((refs / f"{stem}.json").read_text())
+        record = {
+            key: meta[key]
+            for key in (
+                "orbital_basis",
+                "auxiliary_basis",
+                "basis_representation",
+                "atoms_bohr",
+                "aos",
+                "auxiliary_count",
+                "energy_hartree",
+                "forces_hartree_per_bohr",
+                "metric_condition",
+                "stock_effective_rank",
+                "versions",
+            )
+        }
+        (HERE / f"{out}.json").write_text(json.dumps(record, indent=1) + "\n")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

@@ -1,0 +1,158 @@
+"""RHF ``Calculator`` entry point (host mirror of python/generativeqc/calculator.py:146-200).
+
+Only the density-fitted closed-shell RHF path is provided: ``method="rhf"``
+with ``density_fitting`` in ("hip", "cuda", "auto", True).  Coordinates are in
+Bohr; results carry the energy in Hartree and convergence diagnostics.
+``precision`` selects the J/K arithmetic: "fp64" (the reference's DF
+precision), "fp32", "bf16x3", or "mixed" (= bf16x3 iterations followed by FP64
+refinement to the requested tolerances).
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from .basis import Atom, AOBasis, NUMBERS, atoms_from, shells_for
+from .fock import FockBuildSpec, FockPlan
+
+_PRECISIONS = {"fp64": "fp64", "fp32": "fp32", "bf16x3": "bf16x3", "mixed": "bf16x3"}
+
+
+@dataclass(frozen=True)
+class Result:
+    energy: float
+    forces: np.ndarray | None
+    iterations: int
+    converged: bool
+    energy_change: float
+    density_rms: float
+    fock_builds: int
+    density: np.ndarray
+    nbf: int
+    naux: int
+
+
+def _normalise_atoms(atoms) -> tuple:
+    out = []
+    for a in atoms:
+        if isinstance(a, Atom):
+            out.append(a)
+            continue
+        z, pos = a
+        if isinstance(z, str):
+            z = NUMBERS[z.capitalize()]
+        out.append(Atom(int(z), tuple(float(v) for v in pos)))
+    return tuple(out)
+
+
+class Calculator:
+    def __init__(
+        self,
+        method: str = "rhf",
+        basis="cc-pvdz",
+        device: str = "hip",
+        device_id: int = 0,
+        basis_representation: str | None = None,
+        *,
+        density_fitting="hip",
+        auxiliary_basis="cc-pvdz-jkfit",
+        density_fitting_relative_threshold: float = 1.0e-10,
+        density_fitting_memory_budget_bytes: int = 0,
+        max_iterations: int = 100,
+        energy_tolerance: float = 1.0e-10,
+        density_tolerance: float = 1.0e-8,
+        diis_history: int = 8,
+        screening_tolerance: float | None = None,
+        precision: str = "fp64",
+    ):
+        if str(method).lower() != "rhf":
+            raise NotImplementedError("the MI355X engine provides density-fitted RHF only")
+        if density_fitting not in ("hip", "cuda", "auto", True):
+            raise NotImplementedError("only density-fitted J/K is provided (density_fitting='hip')")
+        if device not in ("hip", "cuda", "gpu"):
+            raise NotImplementedError("the engine runs on the HIP device only")
+        if precision not in _PRECISIONS:
+            raise ValueError("precision must be fp64, fp32, bf16x3 or mixed")
+        if not (0.0 < density_fitting_relative_threshold < 1.0):
+            raise ValueError("metric threshold must lie strictly between zero and one")
+        for v in (energy_tolerance, density_tolerance):
+            if not (math.isfinite(v) and v > 0):
+                raise ValueError("tolerances must be finite and positive")
+        self.method = "rhf"
+        self.basis = basis
+        self.auxiliary_basis = auxiliary_basis
+        self.device_id = device_id
+        self.representation = "cartesian" if basis_representation == "cartesian" else "real_spherical"
+        self.metric_threshold = density_fitting_relative_threshold
+        self.budget = density_fitting_memory_budget_bytes
+        self.max_iterations = max_iterations
+        self.energy_tolerance = energy_tolerance
+        self.density_tolerance = density_tolerance
+        self.diis_history = diis_history
+        self.screening_tolerance = 1e-12 if screening_tolerance is None else screening_tolerance
+        self.precision = precision
+
+    def _bases(self, atoms, charge, multiplicity):
+        orbital = AOBasis(
+            atoms,
+            self.basis if isinstance(self.basis, tuple) else shells_for(self.basis, atoms),
+            self.representation,
+            charge,
+            multiplicity,
+        )
+        aux_source = self.auxiliary_basis
+        aux = AOBasis(
+            atoms,
+            aux_source if isinstance(aux_source, tuple) else shells_for(aux_source, atoms),
+            self.representation,
+            charge,
+            multiplicity,
+        )
+        return orbital, aux
+
+    def prepare(self, atoms, *, charge: int = 0, multiplicity: int = 1) -> FockPlan:
+        """Build the device plan (integrals, metric factor, B) for one geometry."""
+        atoms = _normalise_atoms(atoms)
+        orbital, aux = self._bases(atoms, charge, multiplicity)
+        return FockPlan(
+            orbital,
+            FockBuildSpec.hf(),
+            auxiliary=aux,
+            device_id=self.device_id,
+            screening_tolerance=self.screening_tolerance,
+            metric_relative_threshold=self.metric_threshold,
+            device_budget_bytes=self.budget,
+            precision=_PRECISIONS[self.precision],
+        )
+
+    def singlepoint(self, atoms, *, charge: int = 0, multiplicity: int = 1, properties=("energy",)) -> Result:
+        """Energy (Hartree) of one system; forces are not yet provided."""
+        props = tuple(properties) if properties is not None else ("energy",)
+        if "forces" in props:
+            raise NotImplementedError("analytic forces are not yet provided by the HIP engine")
+        with self.prepare(atoms, charge=charge, multiplicity=multiplicity) as plan:
+            r = plan.solve(
+                max_iterations=self.max_iterations,
+                diis_history=self.diis_history,
+                energy_tolerance=self.energy_tolerance,
+                density_tolerance=self.density_tolerance,
+            )
+            shape = plan.shape()
+        return Result(
+            r.energy,
+            None,
+            r.iterations,
+            True,
+            r.energy_change,
+            r.density_rms,
+            r.fock_builds,
+            r.density,
+            shape["nbf"],
+            shape["naux"],
+        )
+
+
+__all__ = ["Calculator", "Result", "atoms_from"]

@@ -1,0 +1,477 @@
+// extern "C" boundary: the generativeqc_* context / system / Fock-plan ABI
+// (reference semantics: src/api/c_api_fock.cpp:22-310, src/api/error.cpp:12-35)
+// and the vibeqc_* extensions declared in include/generativeqc/fock.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "basis.hpp"
+#include "common.hpp"
+#include "generativeqc/fock.h"
+#include "generativeqc/generativeqc.h"
+#include "plan.hpp"
+
+struct generativeqc_context {
+  int device = 0;
+  std::recursive_mutex mutex;
+  std::string detail;
+};
+
+struct generativeqc_system {
+  vqc::System data;
+};
+
+struct generativeqc_fock_plan {
+  std::unique_ptr<vqc::DfPlan> plan;
+  generativeqc_fock_spec requested{};
+  generativeqc_fock_spec resolved{};
+  double screening = 1e-12, threshold = 1e-10;
+  uint64_t budget = 0;
+  std::string detail;
+};
+
+namespace {
+
+// Exception -> status mapping (reference src/api/error.cpp:12-35).
+generativeqc_status map_exception(std::string* detail) {
+  try {
+    throw;
+  } catch (const std::bad_alloc&) {
+    if (detail) *detail = "out of host memory";
+    return GENERATIVEQC_STATUS_OUT_OF_MEMORY;
+  } catch (const vqc::OutOfMemory& e) {
+    if (detail) *detail = e.what();
+    return GENERATIVEQC_STATUS_OUT_OF_MEMORY;
+  } catch (const vqc::NotImplemented& e) {
+    if (detail) *detail = e.what();
+    return GENERATIVEQC_STATUS_NOT_IMPLEMENTED;
+  } catch (const vqc::DeviceError& e) {
+    if (detail) *detail = e.what();
+    return GENERATIVEQC_STATUS_CUDA_ERROR;
+  } catch (const std::invalid_argument& e) {
+    if (detail) *detail = e.what();
+    return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  } catch (const std::exception& e) {
+    if (detail) *detail = e.what();
+    return GENERATIVEQC_STATUS_NUMERICAL_FAILURE;
+  } catch (...) {
+    if (detail) *detail = "unknown native failure";
+    return GENERATIVEQC_STATUS_INTERNAL_ERROR;
+  }
+}
+
+template <class T>
+bool valid_descriptor(const T* d) {
+  return d && d->struct_size == sizeof(T) && d->abi_version == GENERATIVEQC_ABI_VERSION;
+}
+
+void require(bool condition, const char* message) {
+  if (!condition) throw std::invalid_argument(message);
+}
+
+std::size_t buffer_bytes(uint64_t count) {
+  require(count <= std::numeric_limits<std::size_t>::max() / sizeof(double), "output size overflows address space");
+  return static_cast<std::size_t>(count) * sizeof(double);
+}
+
+// Output aliasing would make transactional publication ambiguous
+// (c_api_fock.cpp:104-118).
+void disjoint_outputs(std::initializer_list<std::pair<const void*, std::size_t>> outputs) {
+  std::vector<std::pair<std::uintptr_t, std::uintptr_t>> ranges;
+  for (const auto& [ptr, bytes] : outputs) {
+    if (!ptr) continue;
+    const auto first = reinterpret_cast<std::uintptr_t>(ptr);
+    require(bytes <= std::numeric_limits<std::uintptr_t>::max() - first, "output range overflows address space");
+    ranges.emplace_back(first, first + bytes);
+  }
+  std::sort(ranges.begin(), ranges.end());
+  for (std::size_t i = 1; i < ranges.size(); ++i)
+    require(ranges[i - 1].second <= ranges[i].first, "Fock output buffers overlap");
+}
+
+void finite(const std::vector<double>& v) {
+  for (double x : v)
+    if (!std::isfinite(x)) throw std::runtime_error("nonfinite Fock output");
+}
+
+bool term_ok(const generativeqc_fock_term& t) {
+  return t.present == 0 ||
+         (t.op == GENERATIVEQC_FOCK_FULL_RANGE && t.approximation == GENERATIVEQC_FOCK_DENSITY_FITTED);
+}
+
+void copy_text(char* dst, std::size_t cap, const char* src) {
+  std::strncpy(dst, src, cap - 1);
+  dst[cap - 1] = '\0';
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t generativeqc_get_abi_version(void) { return GENERATIVEQC_ABI_VERSION; }
+
+const char* generativeqc_get_source_identity(void) {
+  return "vibeqc-amd: MI355X (gfx950) HIP DF-JK / RHF engine, hand-written MD integrals + MFMA J/K";
+}
+
+const char* generativeqc_status_message(generativeqc_status status) {
+  switch (status) {
+    case GENERATIVEQC_STATUS_SUCCESS: return "success";
+    case GENERATIVEQC_STATUS_INVALID_ARGUMENT: return "invalid argument";
+    case GENERATIVEQC_STATUS_ABI_MISMATCH: return "ABI mismatch";
+    case GENERATIVEQC_STATUS_NOT_IMPLEMENTED: return "not implemented";
+    case GENERATIVEQC_STATUS_NOT_CONVERGED: return "not converged";
+    case GENERATIVEQC_STATUS_NUMERICAL_FAILURE: return "numerical failure";
+    case GENERATIVEQC_STATUS_CUDA_ERROR: return "device (HIP) error";
+    case GENERATIVEQC_STATUS_OUT_OF_MEMORY: return "out of memory";
+    case GENERATIVEQC_STATUS_INTERNAL_ERROR: return "internal error";
+    case GENERATIVEQC_STATUS_PRECISION_UNAVAILABLE: return "precision provenance unavailable";
+    default: return "unknown status";
+  }
+}
+
+generativeqc_status generativeqc_context_create(const generativeqc_context_descriptor* descriptor,
+                                                generativeqc_context** context) {
+  if (context) *context = nullptr;
+  if (!descriptor || !context) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  if (!valid_descriptor(descriptor)) return GENERATIVEQC_STATUS_ABI_MISMATCH;
+  if (descriptor->backend != GENERATIVEQC_BACKEND_CUDA) return GENERATIVEQC_STATUS_NOT_IMPLEMENTED;
+  try {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return GENERATIVEQC_STATUS_CUDA_ERROR;
+    if (descriptor->device_id < 0 || descriptor->device_id >= count) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+    VQC_HIP(hipSetDevice(descriptor->device_id));
+    auto c = std::make_unique<generativeqc_context>();
+    c->device = descriptor->device_id;
+    *context = c.release();
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(nullptr);
+  }
+}
+
+void generativeqc_context_destroy(generativeqc_context* context) { delete context; }
+
+const char* generativeqc_context_get_last_detail(const generativeqc_context* context) {
+  return context ? context->detail.c_str() : "null context";
+}
+
+const char* generativeqc_context_last_error(const generativeqc_context* context) {
+  return generativeqc_context_get_last_detail(context);
+}
+
+generativeqc_status generativeqc_system_create(generativeqc_context* context,
+                                               const generativeqc_system_descriptor* descriptor,
+                                               generativeqc_system** system) {
+  if (system) *system = nullptr;
+  if (!context || !descriptor || !system) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  if (!valid_descriptor(descriptor)) return GENERATIVEQC_STATUS_ABI_MISMATCH;
+  std::lock_guard<std::recursive_mutex> lock(context->mutex);
+  try {
+    auto s = std::make_unique<generativeqc_system>();
+    const generativeqc_status st = vqc::make_system(*descriptor, s->data, context->detail);
+    if (st != GENERATIVEQC_STATUS_SUCCESS) return st;
+    *system = s.release();
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(&context->detail);
+  }
+}
+
+void generativeqc_system_destroy(generativeqc_system* system) { delete system; }
+
+generativeqc_status generativeqc_fock_plan_create(generativeqc_context* context, const generativeqc_system* system,
+                                                  const generativeqc_system* auxiliary,
+                                                  const generativeqc_fock_spec* spec,
+                                                  const generativeqc_fock_controls* controls,
+                                                  generativeqc_fock_plan** output) {
+  if (output) *output = nullptr;
+  if (!context || !system || !spec || !output) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  if (!valid_descriptor(spec) || (controls && !valid_descriptor(controls))) return GENERATIVEQC_STATUS_ABI_MISMATCH;
+  std::lock_guard<std::recursive_mutex> lock(context->mutex);
+  try {
+    require(spec->coulomb.present == 0 || spec->coulomb.present == 1, "Fock presence must be zero or one");
+    require(spec->exchange.present == 0 || spec->exchange.present == 1, "Fock presence must be zero or one");
+    require(std::isfinite(spec->coulomb.coefficient) && std::isfinite(spec->exchange.coefficient),
+            "Fock coefficients must be finite");
+    const double screening = controls ? controls->screening_tolerance : 1e-12;
+    const double requested = controls ? controls->metric_relative_threshold : 0.0;
+    require(std::isfinite(screening) && screening >= 0.0, "Fock screening tolerance must be finite and nonnegative");
+    require(std::isfinite(requested) && requested >= 0.0 && requested < 1.0,
+            "Fock metric cutoff must be finite and in [0, 1)");
+    if (spec->spin != GENERATIVEQC_FOCK_RESTRICTED)
+      throw vqc::NotImplemented("only restricted (closed-shell) Fock plans are implemented");
+    if (spec->derivative_order != 0)
+      throw vqc::NotImplemented("analytic derivatives are not yet provided by this engine");
+    if (!term_ok(spec->coulomb) || !term_ok(spec->exchange))
+      throw vqc::NotImplemented("only full-range density-fitted J/K terms are implemented");
+    if (!spec->coulomb.present && !spec->exchange.present)
+      throw std::invalid_argument("a Fock plan requires at least one J or K term");
+    vqc::PlanOptions opts;
+    opts.screening = screening;
+    opts.metric_threshold = requested == 0.0 ? 1e-10 : requested;
+    opts.coulomb = spec->coulomb.present != 0;
+    opts.exchange = spec->exchange.present != 0;
+    opts.cj = opts.coulomb ? spec->coulomb.coefficient : 0.0;
+    opts.ck = opts.exchange ? spec->exchange.coefficient : 0.0;
+    auto plan = std::make_unique<generativeqc_fock_plan>();
+    plan->requested = *spec;
+    plan->resolved = *spec;
+    plan->screening = opts.screening;
+    plan->threshold = opts.metric_threshold;
+    plan->budget = controls ? controls->device_budget_bytes : 0;
+    const vqc::System& aux = auxiliary ? auxiliary->data : system->data;
+    plan->plan = std::make_unique<vqc::DfPlan>(context->device, system->data, aux, opts);
+    *output = plan.release();
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(&context->detail);
+  }
+}
+
+void generativeqc_fock_plan_destroy(generativeqc_fock_plan* plan) { delete plan; }
+
+const char* generativeqc_fock_plan_last_error(const generativeqc_fock_plan* plan) {
+  return plan ? plan->detail.c_str() : "null Fock plan";
+}
+
+generativeqc_status generativeqc_fock_plan_evaluate(generativeqc_fock_plan* plan, const double* density,
+                                                    uint64_t density_count, const double* beta, uint64_t beta_count,
+                                                    generativeqc_fock_result* result) {
+  if (!plan || !density || !result) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  if (!valid_descriptor(result)) return GENERATIVEQC_STATUS_ABI_MISMATCH;
+  plan->detail.clear();
+  try {
+    auto& p = *plan->plan;
+    const uint64_t matrix = static_cast<uint64_t>(p.nbf()) * p.nbf();
+    require(density_count == matrix && result->matrix_count == matrix && beta == nullptr && beta_count == 0,
+            "Fock density/output shape or spin mismatch");
+    require(result->gradient ? false : result->gradient_count == 0, "Fock gradient capability/count mismatch");
+    require(result->exchange_beta == nullptr && result->fock_beta == nullptr,
+            "restricted plans have no beta outputs");
+    const auto bytes = buffer_bytes(matrix);
+    disjoint_outputs({{result->coulomb, bytes},
+                      {result->exchange_alpha, bytes},
+                      {result->fock_alpha, bytes},
+                      {result, sizeof(*result)}});
+    const std::vector<double> d(density, density + matrix);
+    const vqc::JkResult jk = p.evaluate(d);
+    const auto& o = p.options();
+    const std::vector<double> h = p.hcore();
+    std::vector<double> fock(matrix);
+    double e1 = 0.0, ej = 0.0, ek = 0.0, cj = 0.0, ck = 0.0;
+    // Compensated sums as in fock_build.cpp:460-491.
+    auto kahan = [](double& sum, double& comp, double v) {
+      const double y = v - comp;
+      const double t = sum + y;
+      comp = (t - sum) - y;
+      sum = t;
+    };
+    double c1 = 0.0;
+    for (uint64_t i = 0; i < matrix; ++i) {
+      fock[i] = h[i] + (o.coulomb ? o.cj * jk.J[i] : 0.0) + (o.exchange ? o.ck * jk.K[i] : 0.0);
+      kahan(e1, c1, d[i] * h[i]);
+      if (o.coulomb) kahan(ej, cj, 0.5 * d[i] * o.cj * jk.J[i]);
+      if (o.exchange) kahan(ek, ck, 0.5 * d[i] * o.ck * jk.K[i]);
+    }
+    const double e2 = ej + ek;
+    if (!std::isfinite(e1) || !std::isfinite(e2)) throw std::runtime_error("nonfinite Fock energy");
+    finite(jk.J);
+    finite(jk.K);
+    finite(fock);
+    if (result->coulomb && o.coulomb) std::copy(jk.J.begin(), jk.J.end(), result->coulomb);
+    if (result->exchange_alpha && o.exchange) std::copy(jk.K.begin(), jk.K.end(), result->exchange_alpha);
+    if (result->fock_alpha) std::copy(fock.begin(), fock.end(), result->fock_alpha);
+    result->energy_one_electron = e1;
+    result->energy_two_electron = e2;
+    result->nuclear_repulsion = p.nuclear_repulsion();
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(&plan->detail);
+  }
+}
+
+generativeqc_status generativeqc_fock_plan_solve(generativeqc_fock_plan* plan,
+                                                 const generativeqc_fock_scf_controls* controls,
+                                                 const double* initial_density, uint64_t initial_density_count,
+                                                 generativeqc_fock_scf_result* output) {
+  if (!plan || !output) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  if (!valid_descriptor(output) || (controls && !valid_descriptor(controls))) return GENERATIVEQC_STATUS_ABI_MISMATCH;
+  plan->detail.clear();
+  try {
+    auto& p = *plan->plan;
+    const uint64_t count = static_cast<uint64_t>(p.nbf()) * p.nbf();
+    require(output->density ? output->density_count == count : output->density_count == 0,
+            "SCF density output shape mismatch");
+    if (output->forces || output->force_count)
+      throw vqc::NotImplemented("analytic forces are not yet provided by this engine");
+    require(initial_density ? initial_density_count == count : initial_density_count == 0,
+            "SCF initial density shape mismatch");
+    disjoint_outputs({{output->density, buffer_bytes(output->density_count)}, {output, sizeof(*output)}});
+    vqc::ScfOptions opts;
+    if (controls) {
+      require(controls->max_iterations > 0 && controls->diis_history > 0 &&
+                  std::isfinite(controls->energy_tolerance) && controls->energy_tolerance > 0 &&
+                  std::isfinite(controls->density_tolerance) && controls->density_tolerance > 0,
+              "SCF counts and finite tolerances must be positive");
+      opts.max_iterations = controls->max_iterations;
+      opts.diis_history = std::min<uint32_t>(controls->diis_history, 32);
+      opts.energy_tolerance = controls->energy_tolerance;
+      opts.density_tolerance = controls->density_tolerance;
+    }
+    std::vector<double> seed;
+    if (initial_density) {
+      seed.assign(initial_density, initial_density + count);
+      finite(seed);
+    }
+    const vqc::ScfOutcome r = p.solve(opts, initial_density ? &seed : nullptr);
+    if (!r.converged) {
+      plan->detail = "Fock SCF did not converge in " + std::to_string(r.iterations) + " iterations";
+      return GENERATIVEQC_STATUS_NOT_CONVERGED;
+    }
+    finite(r.density);
+    if (output->density) std::copy(r.density.begin(), r.density.end(), output->density);
+    output->energy = r.energy;
+    output->energy_change = r.energy_change;
+    output->density_rms = r.density_rms;
+    output->iterations = r.iterations;
+    output->initial_density_used = initial_density != nullptr;
+    output->fock_builds = r.fock_builds;
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(&plan->detail);
+  }
+}
+
+generativeqc_status generativeqc_fock_plan_diagnostic(const generativeqc_fock_plan* plan,
+                                                      generativeqc_fock_diagnostic* diagnostic) {
+  if (!plan || !diagnostic) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  if (!valid_descriptor(diagnostic)) return GENERATIVEQC_STATUS_ABI_MISMATCH;
+  const auto& p = *plan->plan;
+  generativeqc_fock_diagnostic d{};
+  d.struct_size = sizeof(d);
+  d.abi_version = GENERATIVEQC_ABI_VERSION;
+  d.requested = plan->requested;
+  d.resolved = plan->resolved;
+  d.backend = GENERATIVEQC_BACKEND_CUDA;
+  d.resolved_schedule = GENERATIVEQC_FOCK_CUDA_INDEPENDENT;
+  d.source_schedule = GENERATIVEQC_FOCK_CUDA_INDEPENDENT;
+  d.nbf = static_cast<uint64_t>(p.nbf());
+  d.coordinate_count = 3 * p.system().atoms.size();
+  d.device_bytes = p.device_bytes();
+  d.device_budget_bytes = plan->budget;
+  d.auxiliary_rank = static_cast<uint64_t>(p.rank());
+  d.auxiliary_tile = static_cast<uint64_t>(p.naux());
+  d.screening_tolerance = plan->screening;
+  d.metric_relative_threshold = plan->threshold;
+  d.df_streamed = 0;
+  copy_text(d.direct_schedule, sizeof(d.direct_schedule), "none");
+  copy_text(d.df_value_backend, sizeof(d.df_value_backend), "hip-gfx950");
+  copy_text(d.df_value_mapping, sizeof(d.df_value_mapping),
+            p.cholesky() ? "cholesky-packed" : "eigen-cutoff-packed");
+  copy_text(d.df_response_mapping, sizeof(d.df_response_mapping), "none");
+  copy_text(d.one_electron_value_backend, sizeof(d.one_electron_value_backend), "hip-gfx950");
+  copy_text(d.one_electron_value_mapping, sizeof(d.one_electron_value_mapping), "shell-pair");
+  copy_text(d.one_electron_response_mapping, sizeof(d.one_electron_response_mapping), "none");
+  *diagnostic = d;
+  return GENERATIVEQC_STATUS_SUCCESS;
+}
+
+generativeqc_status vibeqc_fock_plan_set_precision(generativeqc_fock_plan* plan, vibeqc_precision precision) {
+  if (!plan) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  plan->detail.clear();
+  try {
+    require(precision == VIBEQC_PRECISION_FP64 || precision == VIBEQC_PRECISION_FP32 ||
+                precision == VIBEQC_PRECISION_BF16X3,
+            "unknown precision mode");
+    plan->plan->set_precision(static_cast<vqc::Precision>(precision));
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(&plan->detail);
+  }
+}
+
+generativeqc_status vibeqc_fock_plan_evaluate_occupied(generativeqc_fock_plan* plan, const double* occupied,
+                                                       uint64_t nbf, uint64_t nocc, double* coulomb,
+                                                       double* exchange) {
+  if (!plan || !occupied) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  plan->detail.clear();
+  try {
+    auto& p = *plan->plan;
+    require(nbf == static_cast<uint64_t>(p.nbf()) && nocc <= nbf, "occupied factor shape mismatch");
+    const uint64_t matrix = nbf * nbf;
+    disjoint_outputs({{coulomb, buffer_bytes(matrix)}, {exchange, buffer_bytes(matrix)}});
+    const std::vector<double> c(occupied, occupied + nbf * nocc);
+    const vqc::JkResult jk = p.evaluate_occupied(c, static_cast<int>(nocc));
+    finite(jk.J);
+    finite(jk.K);
+    if (coulomb) std::copy(jk.J.begin(), jk.J.end(), coulomb);
+    if (exchange) std::copy(jk.K.begin(), jk.K.end(), exchange);
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(&plan->detail);
+  }
+}
+
+generativeqc_status vibeqc_fock_plan_benchmark_occupied(generativeqc_fock_plan* plan, const double* occupied,
+                                                        uint64_t nbf, uint64_t nocc, uint32_t repeat,
+                                                        double* kernel_ms) {
+  if (!plan || !occupied) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  plan->detail.clear();
+  try {
+    auto& p = *plan->plan;
+    require(nbf == static_cast<uint64_t>(p.nbf()) && nocc > 0 && nocc <= nbf, "occupied factor shape mismatch");
+    const std::vector<double> c(occupied, occupied + nbf * nocc);
+    const auto ms = p.benchmark_occupied(c, static_cast<int>(nocc), repeat);
+    if (kernel_ms) std::copy(ms.begin(), ms.end(), kernel_ms);
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(&plan->detail);
+  }
+}
+
+generativeqc_status vibeqc_fock_plan_one_electron(const generativeqc_fock_plan* plan, double* overlap, double* hcore,
+                                                  uint64_t matrix_count) {
+  if (!plan) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  try {
+    const auto& p = *plan->plan;
+    const uint64_t matrix = static_cast<uint64_t>(p.nbf()) * p.nbf();
+    require(matrix_count == matrix, "one-electron output shape mismatch");
+    disjoint_outputs({{overlap, buffer_bytes(matrix)}, {hcore, buffer_bytes(matrix)}});
+    if (overlap) {
+      const auto s = p.overlap();
+      std::copy(s.begin(), s.end(), overlap);
+    }
+    if (hcore) {
+      const auto h = p.hcore();
+      std::copy(h.begin(), h.end(), hcore);
+    }
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(nullptr);
+  }
+}
+
+generativeqc_status vibeqc_fock_plan_shape(const generativeqc_fock_plan* plan, uint64_t* shape6) {
+  if (!plan || !shape6) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  const auto& p = *plan->plan;
+  shape6[0] = static_cast<uint64_t>(p.nbf());
+  shape6[1] = static_cast<uint64_t>(p.naux());
+  shape6[2] = static_cast<uint64_t>(p.npair());
+  shape6[3] = static_cast<uint64_t>(p.rank());
+  shape6[4] = p.cholesky() ? 1 : 0;
+  shape6[5] = static_cast<uint64_t>(p.device_bytes());
+  return GENERATIVEQC_STATUS_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,10 @@
+// Integral kernels of the orbital class (l=1, l=0); compiled as its own
+// translation unit so the classes build in parallel.
+#include "integrals_kernels.hpp"
+
+namespace vqc {
+namespace detail {
+VQC_DEFINE_CLASS(1, 0)
+VQC_DEFINE_METRIC(1)
+}  // namespace detail
+}  // namespace vqc

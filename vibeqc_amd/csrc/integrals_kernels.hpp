@@ -1,0 +1,540 @@
+// Integral kernel templates shared by the per-class translation units
+// integrals_cXY.hip (one per orbital (la, lb) class, compiled in parallel) and
+// the host driver integrals.hip.  See integrals.hip for the algorithm notes.
+#pragma once
+
+#include <cmath>
+
+#include "integrals.hpp"
+
+namespace vqc {
+namespace detail {
+
+__host__ __device__ constexpr int ntuv(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
+__host__ __device__ constexpr int tuv(int t, int u, int v) {
+  const int s = t + u + v;
+  return s * (s + 1) * (s + 2) / 6 + t * (s + 1) - t * (t - 1) / 2 + u;
+}
+__host__ __device__ constexpr int nc(int l) { return (l + 1) * (l + 2) / 2; }
+__host__ __device__ constexpr int cart_lx(int l, int c) {
+  int lx = l;
+  while (c > l - lx) {
+    c -= l - lx + 1;
+    --lx;
+  }
+  return lx;
+}
+__host__ __device__ constexpr int cart_lz(int l, int c) {
+  int lx = l;
+  while (c > l - lx) {
+    c -= l - lx + 1;
+    --lx;
+  }
+  return c;
+}
+__host__ __device__ constexpr int cart_ly(int l, int c) { return l - cart_lx(l, c) - cart_lz(l, c); }
+__host__ __device__ constexpr int cart_index(int lx, int ly, int lz) {
+  const int l = lx + ly + lz;
+  int base = 0;
+  for (int x = l; x > lx; --x) base += l - x + 1;
+  (void)ly;
+  return base + lz;
+}
+// Number of Cartesian e components with la <= |e| <= la+lb, and the offset of degree l.
+__host__ __device__ constexpr int ne_offset(int la, int l) {
+  int n = 0;
+  for (int k = la; k < l; ++k) n += nc(k);
+  return n;
+}
+// Hermite coefficient entries of an auxiliary function: degree lc only
+// (real-spherical) or every degree lc, lc-2, ... (Cartesian).
+__host__ __device__ constexpr int hermite_count(int lc, bool cart) {
+  int n = 0;
+  for (int s = lc; s >= 0; s -= 2) {
+    n += nc(s);
+    if (!cart) break;
+  }
+  return n;
+}
+__host__ __device__ constexpr double binom(int n, int k) {
+  double r = 1.0;
+  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+  return r;
+}
+
+template <int M>
+__device__ __forceinline__ void boys_eval(double T, const double* __restrict__ table, double (&F)[M + 1]) {
+  const double e = exp(-T);
+  if (T < kBoysTmax) {
+    const int k = __double2int_rn(T * (1.0 / kBoysStep));
+    const double dt = k * kBoysStep - T;
+    const double* row = table + k * kBoysCols + M;
+    double f = row[7];
+    f = row[6] + f * dt * (1.0 / 7.0);
+    f = row[5] + f * dt * (1.0 / 6.0);
+    f = row[4] + f * dt * (1.0 / 5.0);
+    f = row[3] + f * dt * (1.0 / 4.0);
+    f = row[2] + f * dt * (1.0 / 3.0);
+    f = row[1] + f * dt * (1.0 / 2.0);
+    f = row[0] + f * dt;
+    F[M] = f;
+#pragma unroll
+    for (int m = M; m >= 1; --m) F[m - 1] = (2.0 * T * F[m] + e) * (1.0 / (2 * m - 1));
+  } else {
+    F[0] = 0.5 * sqrt(M_PI / T);
+    const double h = 0.5 / T;
+#pragma unroll
+    for (int m = 1; m <= M; ++m) F[m] = ((2 * m - 1) * F[m - 1] - e) * h;
+  }
+}
+
+// R_{tuv} = R^{(0)}_{tuv} for t+u+v <= L, in place from the highest degree down.
+template <int L>
+__device__ __forceinline__ void hermite_r(double alpha, double X, double Y, double Z, const double (&F)[L + 1],
+                                          double (&R)[ntuv(L)]) {
+  double pw[L + 1];
+  pw[0] = 1.0;
+#pragma unroll
+  for (int n = 1; n <= L; ++n) pw[n] = pw[n - 1] * (-2.0 * alpha);
+  R[0] = pw[L] * F[L];
+#pragma unroll
+  for (int n = L - 1; n >= 0; --n) {
+#pragma unroll
+    for (int s = L - n; s >= 1; --s) {
+#pragma unroll
+      for (int t = 0; t <= s; ++t) {
+#pragma unroll
+        for (int u = 0; u <= s - t; ++u) {
+          const int v = s - t - u;
+          double val;
+          if (t > 0) {
+            val = X * R[tuv(t - 1, u, v)];
+            if (t > 1) val += (t - 1) * R[tuv(t - 2, u, v)];
+          } else if (u > 0) {
+            val = Y * R[tuv(t, u - 1, v)];
+            if (u > 1) val += (u - 1) * R[tuv(t, u - 2, v)];
+          } else {
+            val = Z * R[tuv(t, u, v - 1)];
+            if (v > 1) val += (v - 1) * R[tuv(t, u, v - 2)];
+          }
+          R[tuv(t, u, v)] = val;
+        }
+      }
+    }
+    R[0] = pw[n] * F[n];
+  }
+}
+
+// One-index Hermite coefficients E^{i,0}_t (i <= N) of (x-A)^i about P.
+template <int N>
+__device__ __forceinline__ void hermite_e0(double pa, double h, double (&E)[N + 1][N + 1]) {
+#pragma unroll
+  for (int i = 0; i <= N; ++i)
+#pragma unroll
+    for (int t = 0; t <= N; ++t) E[i][t] = 0.0;
+  E[0][0] = 1.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int t = 0; t <= i + 1; ++t) {
+      double v = 0.0;
+      if (t > 0) v += h * E[i][t - 1];
+      if (t <= i) v += pa * E[i][t];
+      if (t + 1 <= i) v += (t + 1) * E[i][t + 1];
+      E[i + 1][t] = v;
+    }
+}
+
+struct ThreeCenterArgs {
+  const DevShell* bra_shells;
+  const double* bra_exps;
+  const double* bra_coefs;
+  const DevShell* aux_shells;
+  const double* aux_exps;
+  const double* aux_coefs;
+  const int2* tasks;
+  const AuxFunction* fns;
+  int nfn;
+  const double* hermite;
+  const double* ta;
+  const double* tb;
+  int nfa, nfb;
+  const double* boys;
+  double* out;
+  std::size_t ld;
+  double prim_cutoff;
+};
+
+template <int LA, int LB, int LC, bool METRIC, bool CART>
+__global__ __launch_bounds__(64) void three_center_kernel(const ThreeCenterArgs a) {
+  constexpr int LAB = LA + LB;
+  constexpr int L = LAB + LC;
+  constexpr int NE = ne_offset(LA, LAB + 1);
+  constexpr int NH = hermite_count(LC, CART);
+  constexpr int NA = nc(LA), NB = nc(LB);
+  const int f = blockIdx.y * 64 + threadIdx.x;
+  if (f >= a.nfn) return;
+  const int2 task = a.tasks[blockIdx.x];
+  const DevShell A = a.bra_shells[task.x];
+  const DevShell B = METRIC ? A : a.bra_shells[task.y];
+  const AuxFunction fn = a.fns[f];
+  const DevShell C = a.aux_shells[fn.shell];
+
+  double hc[NH];
+#pragma unroll
+  for (int k = 0; k < NH; ++k) hc[k] = a.hermite[fn.coef_off + k];
+
+  double acc[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) acc[k] = 0.0;
+
+  const double ABx = A.x - B.x, ABy = A.y - B.y, ABz = A.z - B.z;
+  const double AB2 = ABx * ABx + ABy * ABy + ABz * ABz;
+  const int nprim_b = METRIC ? 1 : B.nprim;
+  for (int ia = 0; ia < A.nprim; ++ia) {
+    const double ea = a.bra_exps[A.prim_off + ia];
+    const double ca = a.bra_coefs[A.prim_off + ia];
+    for (int ib = 0; ib < nprim_b; ++ib) {
+      const double eb = METRIC ? 0.0 : a.bra_exps[B.prim_off + ib];
+      const double cb = METRIC ? 1.0 : a.bra_coefs[B.prim_off + ib];
+      const double p = ea + eb;
+      const double ip = 1.0 / p;
+      const double kab = ca * cb * exp(-ea * eb * ip * AB2);
+      if (fabs(kab) < a.prim_cutoff) continue;
+      const double Px = (ea * A.x + eb * B.x) * ip;
+      const double Py = (ea * A.y + eb * B.y) * ip;
+      const double Pz = (ea * A.z + eb * B.z) * ip;
+      for (int ic = 0; ic < C.nprim; ++ic) {
+        const double g = a.aux_exps[C.prim_off + ic];
+        const double cg = a.aux_coefs[C.prim_off + ic];
+        const double q = p + g;
+        const double alpha = p * g / q;
+        const double X = Px - C.x, Y = Py - C.y, Z = Pz - C.z;
+        double F[L + 1];
+        boys_eval<L>(alpha * (X * X + Y * Y + Z * Z), a.boys, F);
+        double R[ntuv(L)];
+        hermite_r<L>(alpha, X, Y, Z, F, R);
+        // Ket: W_{tuv} = sum_k hc_k (2g)^{-(LC+s_k)/2} R_{t+tau_k, u+nu_k, v+phi_k}.
+        const double i2g = 0.5 / g;
+        double scale[LC + 1];
+        scale[0] = 1.0;
+#pragma unroll
+        for (int k = 1; k <= LC; ++k) scale[k] = scale[k - 1] * i2g;
+        double W[ntuv(LAB)];
+#pragma unroll
+        for (int s = 0; s <= LAB; ++s)
+#pragma unroll
+          for (int t = 0; t <= s; ++t)
+#pragma unroll
+            for (int u = 0; u <= s - t; ++u) {
+              const int v = s - t - u;
+              double w = 0.0;
+              int k = 0;
+#pragma unroll
+              for (int d = LC; d >= 0; d -= 2) {
+#pragma unroll
+                for (int tt = 0; tt <= d; ++tt)
+#pragma unroll
+                  for (int uu = 0; uu <= d - tt; ++uu) {
+                    const int vv = d - tt - uu;
+                    if (k < NH) w += hc[k] * scale[(LC + d) / 2] * R[tuv(t + tt, u + uu, v + vv)];
+                    ++k;
+                  }
+                if (!CART) break;
+              }
+              W[tuv(t, u, v)] = w;
+            }
+        const double pref = 2.0 * 17.493418327624862 /* pi^{5/2} */ / (p * g * sqrt(q)) * kab * cg;
+        double Ex[LAB + 1][LAB + 1], Ey[LAB + 1][LAB + 1], Ez[LAB + 1][LAB + 1];
+        hermite_e0<LAB>(Px - A.x, 0.5 * ip, Ex);
+        hermite_e0<LAB>(Py - A.y, 0.5 * ip, Ey);
+        hermite_e0<LAB>(Pz - A.z, 0.5 * ip, Ez);
+#pragma unroll
+        for (int le = LA; le <= LAB; ++le)
+#pragma unroll
+          for (int c = 0; c < nc(le); ++c) {
+            const int ex = cart_lx(le, c), ey = cart_ly(le, c), ez = cart_lz(le, c);
+            double s = 0.0;
+#pragma unroll
+            for (int t = 0; t <= ex; ++t) {
+              double su = 0.0;
+#pragma unroll
+              for (int u = 0; u <= ey; ++u) {
+                double sv = 0.0;
+#pragma unroll
+                for (int v = 0; v <= ez; ++v) sv += Ez[ez][v] * W[tuv(t, u, v)];
+                su += Ey[ey][u] * sv;
+              }
+              s += Ex[ex][t] * su;
+            }
+            acc[ne_offset(LA, le) + c] += pref * s;
+          }
+      }
+    }
+  }
+
+  // Horizontal recurrence to Cartesian (ab|, then public AOs.
+  double cart[NA][NB];
+#pragma unroll
+  for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib) {
+      const int ax = cart_lx(LA, ia), ay = cart_ly(LA, ia), az = cart_lz(LA, ia);
+      const int bx = cart_lx(LB, ib), by = cart_ly(LB, ib), bz = cart_lz(LB, ib);
+      double v = 0.0;
+#pragma unroll
+      for (int kx = 0; kx <= bx; ++kx)
+#pragma unroll
+        for (int ky = 0; ky <= by; ++ky)
+#pragma unroll
+          for (int kz = 0; kz <= bz; ++kz) {
+            double w = binom(bx, kx) * binom(by, ky) * binom(bz, kz);
+#pragma unroll
+            for (int r = 0; r < bx - kx; ++r) w *= ABx;
+#pragma unroll
+            for (int r = 0; r < by - ky; ++r) w *= ABy;
+#pragma unroll
+            for (int r = 0; r < bz - kz; ++r) w *= ABz;
+            const int le = LA + kx + ky + kz;
+            v += w * acc[ne_offset(LA, le) + cart_index(ax + kx, ay + ky, az + kz)];
+          }
+      cart[ia][ib] = v;
+    }
+
+  double half[NA][NB];  // [public a][cartesian b]
+#pragma unroll
+  for (int ma = 0; ma < NA; ++ma)
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib) {
+      double v = 0.0;
+      if (ma < a.nfa) {
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia) v += a.ta[ma * NA + ia] * cart[ia][ib];
+      }
+      half[ma][ib] = v;
+    }
+#pragma unroll
+  for (int ma = 0; ma < NA; ++ma) {
+    if (ma >= a.nfa) break;
+#pragma unroll
+    for (int mb = 0; mb < NB; ++mb) {
+      if (mb >= a.nfb) break;
+      double v = 0.0;
+#pragma unroll
+      for (int ib = 0; ib < NB; ++ib) v += a.tb[mb * NB + ib] * half[ma][ib];
+      if (METRIC) {
+        a.out[static_cast<std::size_t>(A.fn_off + ma) * a.ld + fn.index] = v;
+      } else {
+        const int mu = A.fn_off + ma, nu = B.fn_off + mb;
+        if (task.x == task.y && nu > mu) continue;
+        const std::size_t hi = mu > nu ? mu : nu, lo = mu > nu ? nu : mu;
+        a.out[(hi * (hi + 1) / 2 + lo) * a.ld + fn.index] = v;
+      }
+    }
+  }
+}
+
+struct OneElectronArgs {
+  const DevShell* shells;
+  const double* exps;
+  const double* coefs;
+  const int2* tasks;
+  int ntask;
+  const double4* atoms;  // x, y, z, Z
+  int natom;
+  const double* ta;
+  const double* tb;
+  int nfa, nfb;
+  const double* boys;
+  double* S;
+  double* T;
+  double* V;
+  int n;
+};
+
+template <int LA, int LB>
+__global__ __launch_bounds__(64) void one_electron_kernel(const OneElectronArgs a) {
+  constexpr int NA = nc(LA), NB = nc(LB), L = LA + LB;
+  constexpr int JB = LB + 2, TD = LA + JB + 1;
+  const int id = blockIdx.x * 64 + threadIdx.x;
+  if (id >= a.ntask) return;
+  const int2 task = a.tasks[id];
+  const DevShell A = a.shells[task.x];
+  const DevShell B = a.shells[task.y];
+  double s[NA][NB], k[NA][NB], v[NA][NB];
+#pragma unroll
+  for (int i = 0; i < NA; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) s[i][j] = k[i][j] = v[i][j] = 0.0;
+  const double ABx = A.x - B.x, ABy = A.y - B.y, ABz = A.z - B.z;
+  const double AB2 = ABx * ABx + ABy * ABy + ABz * ABz;
+  for (int ia = 0; ia < A.nprim; ++ia) {
+    const double ea = a.exps[A.prim_off + ia], ca = a.coefs[A.prim_off + ia];
+    for (int ib = 0; ib < B.nprim; ++ib) {
+      const double eb = a.exps[B.prim_off + ib], cb = a.coefs[B.prim_off + ib];
+      const double p = ea + eb, ip = 1.0 / p, h = 0.5 * ip;
+      const double kab = ca * cb * exp(-ea * eb * ip * AB2);
+      const double P[3] = {(ea * A.x + eb * B.x) * ip, (ea * A.y + eb * B.y) * ip, (ea * A.z + eb * B.z) * ip};
+      const double PA[3] = {P[0] - A.x, P[1] - A.y, P[2] - A.z};
+      const double PB[3] = {P[0] - B.x, P[1] - B.y, P[2] - B.z};
+      // Two-index Hermite tables E[d][i][j][t] for i <= LA, j <= LB + 2.
+      double E[3][LA + 1][JB + 1][TD];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+#pragma unroll
+        for (int i = 0; i <= LA; ++i)
+#pragma unroll
+          for (int j = 0; j <= JB; ++j)
+#pragma unroll
+            for (int t = 0; t < TD; ++t) E[d][i][j][t] = 0.0;
+        E[d][0][0][0] = 1.0;
+#pragma unroll
+        for (int i = 0; i < LA; ++i)
+#pragma unroll
+          for (int t = 0; t <= i + 1; ++t) {
+            double val = 0.0;
+            if (t > 0) val += h * E[d][i][0][t - 1];
+            if (t <= i) val += PA[d] * E[d][i][0][t];
+            if (t + 1 <= i) val += (t + 1) * E[d][i][0][t + 1];
+            E[d][i + 1][0][t] = val;
+          }
+#pragma unroll
+        for (int i = 0; i <= LA; ++i)
+#pragma unroll
+          for (int j = 0; j < JB; ++j)
+#pragma unroll
+            for (int t = 0; t <= i + j + 1; ++t) {
+              double val = 0.0;
+              if (t > 0) val += h * E[d][i][j][t - 1];
+              if (t <= i + j) val += PB[d] * E[d][i][j][t];
+              if (t + 1 <= i + j) val += (t + 1) * E[d][i][j][t + 1];
+              E[d][i][j + 1][t] = val;
+            }
+      }
+      const double s1 = sqrt(M_PI * ip);
+#pragma unroll
+      for (int i = 0; i < NA; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+          const int ai[3] = {cart_lx(LA, i), cart_ly(LA, i), cart_lz(LA, i)};
+          const int bj[3] = {cart_lx(LB, j), cart_ly(LB, j), cart_lz(LB, j)};
+          double sd[3], kd[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            sd[d] = E[d][ai[d]][bj[d]][0] * s1;
+            double kv = -2.0 * eb * eb * E[d][ai[d]][bj[d] + 2][0] * s1 + eb * (2 * bj[d] + 1) * sd[d];
+            if (bj[d] >= 2) kv -= 0.5 * bj[d] * (bj[d] - 1) * E[d][ai[d]][bj[d] - 2][0] * s1;
+            kd[d] = kv;
+          }
+          s[i][j] += kab * sd[0] * sd[1] * sd[2];
+          k[i][j] += kab * (kd[0] * sd[1] * sd[2] + sd[0] * kd[1] * sd[2] + sd[0] * sd[1] * kd[2]);
+        }
+      for (int c = 0; c < a.natom; ++c) {
+        const double4 at = a.atoms[c];
+        const double X = P[0] - at.x, Y = P[1] - at.y, Z = P[2] - at.z;
+        double F[L + 1];
+        boys_eval<L>(p * (X * X + Y * Y + Z * Z), a.boys, F);
+        double R[ntuv(L)];
+        hermite_r<L>(p, X, Y, Z, F, R);
+        const double pref = -at.w * 2.0 * M_PI * ip * kab;
+#pragma unroll
+        for (int i = 0; i < NA; ++i)
+#pragma unroll
+          for (int j = 0; j < NB; ++j) {
+            const int ai[3] = {cart_lx(LA, i), cart_ly(LA, i), cart_lz(LA, i)};
+            const int bj[3] = {cart_lx(LB, j), cart_ly(LB, j), cart_lz(LB, j)};
+            double acc = 0.0;
+#pragma unroll
+            for (int t = 0; t <= ai[0] + bj[0]; ++t)
+#pragma unroll
+              for (int u = 0; u <= ai[1] + bj[1]; ++u)
+#pragma unroll
+                for (int w = 0; w <= ai[2] + bj[2]; ++w)
+                  acc += E[0][ai[0]][bj[0]][t] * E[1][ai[1]][bj[1]][u] * E[2][ai[2]][bj[2]][w] * R[tuv(t, u, w)];
+            v[i][j] += pref * acc;
+          }
+      }
+    }
+  }
+  // Public AOs: out = TA x TB^T, written to both triangles.
+  for (int ma = 0; ma < a.nfa; ++ma)
+    for (int mb = 0; mb < a.nfb; ++mb) {
+      double os = 0.0, ok = 0.0, ov = 0.0;
+#pragma unroll
+      for (int i = 0; i < NA; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+          const double w = a.ta[ma * NA + i] * a.tb[mb * NB + j];
+          os += w * s[i][j];
+          ok += w * k[i][j];
+          ov += w * v[i][j];
+        }
+      const std::size_t mu = A.fn_off + ma, nu = B.fn_off + mb;
+      a.S[mu * a.n + nu] = os;
+      a.S[nu * a.n + mu] = os;
+      a.T[mu * a.n + nu] = ok;
+      a.T[nu * a.n + mu] = ok;
+      a.V[mu * a.n + nu] = ov;
+      a.V[nu * a.n + mu] = ov;
+    }
+}
+
+template <int LA, int LB, int LC, bool METRIC, bool CART>
+void launch_3c(const ThreeCenterArgs& args, int ntask, hipStream_t s) {
+  if (ntask == 0 || args.nfn == 0) return;
+  dim3 grid(ntask, (args.nfn + 63) / 64);
+  hipLaunchKernelGGL((three_center_kernel<LA, LB, LC, METRIC, CART>), grid, dim3(64), 0, s, args);
+  VQC_LAUNCH("three_center_kernel");
+}
+
+template <int LA, int LB, bool METRIC>
+void dispatch_lc(int lc, bool cart, const ThreeCenterArgs& args, int ntask, hipStream_t s) {
+  switch (lc) {
+    case 0: launch_3c<LA, LB, 0, METRIC, false>(args, ntask, s); break;
+    case 1: launch_3c<LA, LB, 1, METRIC, false>(args, ntask, s); break;
+    case 2:
+      if (cart) launch_3c<LA, LB, 2, METRIC, true>(args, ntask, s);
+      else launch_3c<LA, LB, 2, METRIC, false>(args, ntask, s);
+      break;
+    case 3:
+      if (cart) launch_3c<LA, LB, 3, METRIC, true>(args, ntask, s);
+      else launch_3c<LA, LB, 3, METRIC, false>(args, ntask, s);
+      break;
+    default: throw NotImplemented("auxiliary angular momentum above f");
+  }
+}
+
+template <int LA, int LB>
+void launch_1e(const OneElectronArgs& args, hipStream_t s) {
+  if (args.ntask == 0) return;
+  hipLaunchKernelGGL((one_electron_kernel<LA, LB>), dim3((args.ntask + 63) / 64), dim3(64), 0, s, args);
+  VQC_LAUNCH("one_electron_kernel");
+}
+
+
+// Per-class entry points, defined in integrals_cXY.hip.
+#define VQC_DECLARE_CLASS(LA, LB)                                                                  \
+  void dispatch_3c_##LA##LB(int lc, bool cart, const ThreeCenterArgs& a, int ntask, hipStream_t s); \
+  void dispatch_1e_##LA##LB(const OneElectronArgs& a, hipStream_t s);
+#define VQC_DEFINE_CLASS(LA, LB)                                                                    \
+  void dispatch_3c_##LA##LB(int lc, bool cart, const ThreeCenterArgs& a, int ntask, hipStream_t s) { \
+    dispatch_lc<LA, LB, false>(lc, cart, a, ntask, s);                                               \
+  }                                                                                                 \
+  void dispatch_1e_##LA##LB(const OneElectronArgs& a, hipStream_t s) { launch_1e<LA, LB>(a, s); }
+#define VQC_DEFINE_METRIC(LA)                                                                        \
+  void dispatch_metric_##LA(int lc, bool cart, const ThreeCenterArgs& a, int ntask, hipStream_t s) { \
+    dispatch_lc<LA, 0, true>(lc, cart, a, ntask, s);                                                 \
+  }
+VQC_DECLARE_CLASS(0, 0)
+VQC_DECLARE_CLASS(1, 0)
+VQC_DECLARE_CLASS(1, 1)
+VQC_DECLARE_CLASS(2, 0)
+VQC_DECLARE_CLASS(2, 1)
+VQC_DECLARE_CLASS(2, 2)
+void dispatch_metric_0(int lc, bool cart, const ThreeCenterArgs& a, int ntask, hipStream_t s);
+void dispatch_metric_1(int lc, bool cart, const ThreeCenterArgs& a, int ntask, hipStream_t s);
+void dispatch_metric_2(int lc, bool cart, const ThreeCenterArgs& a, int ntask, hipStream_t s);
+void dispatch_metric_3(int lc, bool cart, const ThreeCenterArgs& a, int ntask, hipStream_t s);
+
+}  // namespace detail
+}  // namespace vqc

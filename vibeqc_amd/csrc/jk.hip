@@ -1,0 +1,619 @@
+// J and K contraction kernels for gfx950 (see jk.hpp for the data layout).
+//
+// MFMA usage:
+//   FP64   v_mfma_f64_16x16x4_f64   (A/B one element per lane, K = 4)
+//   FP32   v_mfma_f32_16x16x4_f32   (same operand layout, FP32 accumulation)
+//   BF16X3 v_mfma_f32_16x16x32_bf16 (8 bf16 per lane, K = 32); each FP32
+//          operand x is split as hi = bf16(x), lo = bf16(x - hi) and the
+//          product is accumulated as hi*hi + hi*lo + lo*hi in FP32.
+// The 16x16 accumulator holds D[4*(lane/16) + r][lane % 16] in register r.
+#include "jk.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace vqc {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+template <class T>
+struct AccOf;
+template <>
+struct AccOf<double> {
+  using type = d4;
+};
+template <>
+struct AccOf<float> {
+  using type = f4;
+};
+
+__device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma(float a, float b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma(bf8 a, bf8 b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ std::size_t packed_index(int mu, int nu) {
+  const std::size_t hi = mu > nu ? mu : nu, lo = mu > nu ? nu : mu;
+  return hi * (hi + 1) / 2 + lo;
+}
+
+__device__ __forceinline__ void unpack_pair(std::size_t p, int& mu, int& nu) {
+  int m = static_cast<int>((sqrt(8.0 * static_cast<double>(p) + 1.0) - 1.0) * 0.5);
+  while (static_cast<std::size_t>(m) * (m + 1) / 2 > p) --m;
+  while (static_cast<std::size_t>(m + 1) * (m + 2) / 2 <= p) ++m;
+  mu = m;
+  nu = static_cast<int>(p - static_cast<std::size_t>(m) * (m + 1) / 2);
+}
+
+__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
+  hi = static_cast<__bf16>(x);
+  lo = static_cast<__bf16>(x - static_cast<float>(hi));
+}
+
+// ---------------------------------------------------------------------------
+// J
+// ---------------------------------------------------------------------------
+
+__global__ void pack_density_kernel(const double* __restrict__ D, int n, double* __restrict__ dp) {
+  const std::size_t p = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= static_cast<std::size_t>(n) * (n + 1) / 2) return;
+  int mu, nu;
+  unpack_pair(p, mu, nu);
+  const std::size_t a = static_cast<std::size_t>(mu) * n + nu, b = static_cast<std::size_t>(nu) * n + mu;
+  dp[p] = mu == nu ? D[a] : D[a] + D[b];
+}
+
+constexpr int kRhoThreads = 256;
+
+template <class T, int V>
+__global__ __launch_bounds__(kRhoThreads) void coulomb_rho_kernel(const T* __restrict__ B,
+                                                                  const double* __restrict__ dp,
+                                                                  std::size_t npair, int ldq, std::size_t chunk,
+                                                                  double* __restrict__ partial) {
+  const int q = (blockIdx.x * kRhoThreads + threadIdx.x) * V;
+  if (q >= ldq) return;
+  const std::size_t p0 = static_cast<std::size_t>(blockIdx.y) * chunk;
+  const std::size_t p1 = p0 + chunk < npair ? p0 + chunk : npair;
+  double acc[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) acc[v] = 0.0;
+  for (std::size_t p = p0; p < p1; ++p) {
+    const double w = dp[p];
+    const T* row = B + p * ldq + q;
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] += w * static_cast<double>(row[v]);
+  }
+  double* out = partial + static_cast<std::size_t>(blockIdx.y) * ldq + q;
+#pragma unroll
+  for (int v = 0; v < V; ++v) out[v] = acc[v];
+}
+
+__global__ void sum_rows_kernel(const double* __restrict__ partial, int rows, int cols, double* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  double s = 0.0;
+  for (int r = 0; r < rows; ++r) s += partial[static_cast<std::size_t>(r) * cols + c];
+  out[c] = s;
+}
+
+template <class T, int V>
+__global__ __launch_bounds__(256) void coulomb_matrix_kernel(const T* __restrict__ B, const double* __restrict__ rho,
+                                                            std::size_t npair, int ldq, int n,
+                                                            double* __restrict__ J) {
+  extern __shared__ double srho[];
+  for (int q = threadIdx.x; q < ldq; q += blockDim.x) srho[q] = rho[q];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const std::size_t wave = static_cast<std::size_t>(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const std::size_t nwave = static_cast<std::size_t>(gridDim.x) * (blockDim.x / 64);
+  for (std::size_t p = wave; p < npair; p += nwave) {
+    const T* row = B + p * ldq;
+    double s = 0.0;
+    for (int q = lane * V; q < ldq; q += 64 * V) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) s += static_cast<double>(row[q + v]) * srho[q + v];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) {
+      int mu, nu;
+      unpack_pair(p, mu, nu);
+      J[static_cast<std::size_t>(mu) * n + nu] = s;
+      J[static_cast<std::size_t>(nu) * n + mu] = s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K, step 1: U[mu][i][Q] = sum_nu B[(mu,nu)][Q] C[nu][i]
+// Block: 256 threads = 4 waves stacked along Q; tile 128 (Q) x 16*NI (i).
+// ---------------------------------------------------------------------------
+
+template <class T, int NI>
+__global__ __launch_bounds__(256) void occupied_u_kernel(const T* __restrict__ B, const T* __restrict__ C,
+                                                        T* __restrict__ U, int n, int ldq, int ldc, int r,
+                                                        int nqb) {
+  using Acc = typename AccOf<T>::type;
+  constexpr int BM = 128, BK = 16, BN = 16 * NI;
+  __shared__ T Xs[BK][BM];
+  __shared__ T Ys[BK][BN];
+  const int mu = blockIdx.x / nqb;
+  const int q0 = (blockIdx.x % nqb) * BM;
+  const int i0 = blockIdx.y * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  Acc acc[2][NI];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < NI; ++b) acc[a][b] = Acc{};
+  const int xk = tid >> 4, xm = (tid & 15) * 8;
+  for (int nu0 = 0; nu0 < n; nu0 += BK) {
+    {
+      const int nu = nu0 + xk;
+      if (nu < n) {
+        const T* src = B + packed_index(mu, nu) * ldq + q0 + xm;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) Xs[xk][xm + j] = src[j];
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) Xs[xk][xm + j] = T(0);
+      }
+    }
+    for (int e = tid; e < BK * BN; e += 256) {
+      const int k = e / BN, c = e % BN;
+      const int nu = nu0 + k;
+      Ys[k][c] = nu < n ? C[static_cast<std::size_t>(nu) * ldc + i0 + c] : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int k = kk + (lane >> 4);
+      T a0 = Xs[k][wave * 32 + (lane & 15)];
+      T a1 = Xs[k][wave * 32 + 16 + (lane & 15)];
+#pragma unroll
+      for (int b = 0; b < NI; ++b) {
+        const T bv = Ys[k][b * 16 + (lane & 15)];
+        acc[0][b] = mfma(a0, bv, acc[0][b]);
+        acc[1][b] = mfma(a1, bv, acc[1][b]);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < NI; ++b) {
+      const int i = i0 + b * 16 + (lane & 15);
+      if (i >= r) continue;
+      const int q = q0 + wave * 32 + a * 16 + 4 * (lane >> 4);
+      T* dst = U + (static_cast<std::size_t>(mu) * r + i) * ldq + q;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) dst[v] = acc[a][b][v];
+    }
+}
+
+// BF16X3 variant: B and C are FP32; Ct_hi/Ct_lo are C^T split into bf16
+// halves (ldc rows of ldct K-contiguous entries, zero beyond n).
+template <int NI>
+__global__ __launch_bounds__(256) void occupied_u_bf16x3_kernel(const float* __restrict__ B,
+                                                               const __bf16* __restrict__ Cth,
+                                                               const __bf16* __restrict__ Ctl,
+                                                               float* __restrict__ U, int n, int ldq, int ldct,
+                                                               int r, int nqb) {
+  constexpr int BM = 128, BK = 32, BN = 16 * NI, PAD = 8;
+  __shared__ __bf16 Xh[BM][BK + PAD], Xl[BM][BK + PAD];
+  __shared__ __bf16 Yh[BN][BK + PAD], Yl[BN][BK + PAD];
+  const int mu = blockIdx.x / nqb;
+  const int q0 = (blockIdx.x % nqb) * BM;
+  const int i0 = blockIdx.y * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  f4 acc[2][NI];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < NI; ++b) acc[a][b] = f4{};
+  const int xk = tid >> 3, xm = (tid & 7) * 16;
+  for (int nu0 = 0; nu0 < n; nu0 += BK) {
+    {
+      const int nu = nu0 + xk;
+      if (nu < n) {
+        const float* src = B + packed_index(mu, nu) * ldq + q0 + xm;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          __bf16 h, l;
+          split_bf16(src[j], h, l);
+          Xh[xm + j][xk] = h;
+          Xl[xm + j][xk] = l;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          Xh[xm + j][xk] = static_cast<__bf16>(0.0f);
+          Xl[xm + j][xk] = static_cast<__bf16>(0.0f);
+        }
+      }
+    }
+    for (int e = tid; e < BN * 4; e += 256) {
+      const int row = e >> 2, kc = (e & 3) * 8;
+      const std::size_t off = static_cast<std::size_t>(i0 + row) * ldct + nu0 + kc;
+      *reinterpret_cast<bf8*>(&Yh[row][kc]) = *reinterpret_cast<const bf8*>(Cth + off);
+      *reinterpret_cast<bf8*>(&Yl[row][kc]) = *reinterpret_cast<const bf8*>(Ctl + off);
+    }
+    __syncthreads();
+    const int kq = 8 * (lane >> 4);
+    const bf8 ah0 = *reinterpret_cast<const bf8*>(&Xh[wave * 32 + (lane & 15)][kq]);
+    const bf8 al0 = *reinterpret_cast<const bf8*>(&Xl[wave * 32 + (lane & 15)][kq]);
+    const bf8 ah1 = *reinterpret_cast<const bf8*>(&Xh[wave * 32 + 16 + (lane & 15)][kq]);
+    const bf8 al1 = *reinterpret_cast<const bf8*>(&Xl[wave * 32 + 16 + (lane & 15)][kq]);
+#pragma unroll
+    for (int b = 0; b < NI; ++b) {
+      const bf8 bh = *reinterpret_cast<const bf8*>(&Yh[b * 16 + (lane & 15)][kq]);
+      const bf8 bl = *reinterpret_cast<const bf8*>(&Yl[b * 16 + (lane & 15)][kq]);
+      acc[0][b] = mfma(ah0, bh, acc[0][b]);
+      acc[0][b] = mfma(ah0, bl, acc[0][b]);
+      acc[0][b] = mfma(al0, bh, acc[0][b]);
+      acc[1][b] = mfma(ah1, bh, acc[1][b]);
+      acc[1][b] = mfma(ah1, bl, acc[1][b]);
+      acc[1][b] = mfma(al1, bh, acc[1][b]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < NI; ++b) {
+      const int i = i0 + b * 16 + (lane & 15);
+      if (i >= r) continue;
+      const int q = q0 + wave * 32 + a * 16 + 4 * (lane >> 4);
+      float* dst = U + (static_cast<std::size_t>(mu) * r + i) * ldq + q;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) dst[v] = acc[a][b][v];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K, step 2: lower-triangular split-K SYRK over W (n x kd, K-contiguous rows).
+// Block: 256 threads = 2x2 waves, tile 64 x 64, partial results per split.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void tri_tile(int t, int& bi, int& bj) {
+  int i = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while (i * (i + 1) / 2 > t) --i;
+  while ((i + 1) * (i + 2) / 2 <= t) ++i;
+  bi = i;
+  bj = t - i * (i + 1) / 2;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void exchange_syrk_kernel(const T* __restrict__ W, std::size_t kd, int n,
+                                                           std::size_t kbegin, std::size_t kend,
+                                                           std::size_t kchunk, double sign,
+                                                           double* __restrict__ partial) {
+  using Acc = typename AccOf<T>::type;
+  constexpr int BK = 16, BT = 64, PAD = 1;
+  __shared__ T As[BK][BT + PAD];
+  __shared__ T Bs[BK][BT + PAD];
+  int bi, bj;
+  tri_tile(blockIdx.x, bi, bj);
+  const int m0 = bi * BT, l0 = bj * BT;
+  const std::size_t k0 = kbegin + static_cast<std::size_t>(blockIdx.y) * kchunk;
+  const std::size_t k1 = k0 + kchunk < kend ? k0 + kchunk : kend;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  Acc acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = Acc{};
+  const int lr = tid >> 2, lk = (tid & 3) * 4;
+  for (std::size_t kb = k0; kb < k1; kb += BK) {
+    {
+      const int mrow = m0 + lr, lrow = l0 + lr;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const std::size_t k = kb + lk + j;
+        As[lk + j][lr] = (mrow < n && k < k1) ? W[static_cast<std::size_t>(mrow) * kd + k] : T(0);
+        Bs[lk + j][lr] = (lrow < n && k < k1) ? W[static_cast<std::size_t>(lrow) * kd + k] : T(0);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int k = kk + (lane >> 4);
+      T a[2], b[2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        a[x] = As[k][wm * 32 + x * 16 + (lane & 15)];
+        b[x] = Bs[k][wn * 32 + x * 16 + (lane & 15)];
+      }
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = mfma(a[x], b[y], acc[x][y]);
+    }
+    __syncthreads();
+  }
+  double* out = partial + static_cast<std::size_t>(blockIdx.y) * n * n;
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+      const int lam = l0 + wn * 32 + y * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int mu = m0 + wm * 32 + x * 16 + 4 * (lane >> 4) + v;
+        if (mu < n && lam < n) out[static_cast<std::size_t>(mu) * n + lam] = sign * static_cast<double>(acc[x][y][v]);
+      }
+    }
+}
+
+__global__ __launch_bounds__(256) void exchange_syrk_bf16x3_kernel(const float* __restrict__ W, std::size_t kd,
+                                                                  int n, std::size_t kbegin, std::size_t kend,
+                                                                  std::size_t kchunk, double sign,
+                                                                  double* __restrict__ partial) {
+  constexpr int BK = 32, BT = 64, PAD = 8;
+  __shared__ __bf16 Ah[BT][BK + PAD], Al[BT][BK + PAD];
+  __shared__ __bf16 Bh[BT][BK + PAD], Bl[BT][BK + PAD];
+  int bi, bj;
+  tri_tile(blockIdx.x, bi, bj);
+  const int m0 = bi * BT, l0 = bj * BT;
+  const std::size_t k0 = kbegin + static_cast<std::size_t>(blockIdx.y) * kchunk;
+  const std::size_t k1 = k0 + kchunk < kend ? k0 + kchunk : kend;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  f4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = f4{};
+  const int lr = tid >> 2, lk = (tid & 3) * 8;
+  for (std::size_t kb = k0; kb < k1; kb += BK) {
+    {
+      const int mrow = m0 + lr, lrow = l0 + lr;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const std::size_t k = kb + lk + j;
+        const float a = (mrow < n && k < k1) ? W[static_cast<std::size_t>(mrow) * kd + k] : 0.0f;
+        const float b = (lrow < n && k < k1) ? W[static_cast<std::size_t>(lrow) * kd + k] : 0.0f;
+        split_bf16(a, Ah[lr][lk + j], Al[lr][lk + j]);
+        split_bf16(b, Bh[lr][lk + j], Bl[lr][lk + j]);
+      }
+    }
+    __syncthreads();
+    const int kq = 8 * (lane >> 4);
+    bf8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+      ah[x] = *reinterpret_cast<const bf8*>(&Ah[wm * 32 + x * 16 + (lane & 15)][kq]);
+      al[x] = *reinterpret_cast<const bf8*>(&Al[wm * 32 + x * 16 + (lane & 15)][kq]);
+      bh[x] = *reinterpret_cast<const bf8*>(&Bh[wn * 32 + x * 16 + (lane & 15)][kq]);
+      bl[x] = *reinterpret_cast<const bf8*>(&Bl[wn * 32 + x * 16 + (lane & 15)][kq]);
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y) {
+        acc[x][y] = mfma(ah[x], bh[y], acc[x][y]);
+        acc[x][y] = mfma(ah[x], bl[y], acc[x][y]);
+        acc[x][y] = mfma(al[x], bh[y], acc[x][y]);
+      }
+    __syncthreads();
+  }
+  double* out = partial + static_cast<std::size_t>(blockIdx.y) * n * n;
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+      const int lam = l0 + wn * 32 + y * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int mu = m0 + wm * 32 + x * 16 + 4 * (lane >> 4) + v;
+        if (mu < n && lam < n) out[static_cast<std::size_t>(mu) * n + lam] = sign * static_cast<double>(acc[x][y][v]);
+      }
+    }
+}
+
+__global__ void reduce_exchange_kernel(const double* __restrict__ partial, int slices, int n,
+                                       double* __restrict__ K) {
+  const std::size_t e = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<std::size_t>(n) * n) return;
+  const int mu = static_cast<int>(e / n), lam = static_cast<int>(e % n);
+  if (lam > mu) return;
+  double s = 0.0;
+  for (int k = 0; k < slices; ++k) s += partial[static_cast<std::size_t>(k) * n * n + e];
+  K[static_cast<std::size_t>(mu) * n + lam] = s;
+  K[static_cast<std::size_t>(lam) * n + mu] = s;
+}
+
+__global__ void convert_kernel(const double* __restrict__ in, float* __restrict__ out, std::size_t count) {
+  const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const std::size_t stride = static_cast<std::size_t>(gridDim.x) * blockDim.x;
+  for (std::size_t k = i; k < count; k += stride) out[k] = static_cast<float>(in[k]);
+}
+
+// C (n x ldc, FP64) -> C^T split: hi/lo bf16, ldc rows x ldct columns (zero beyond n).
+__global__ void split_transpose_kernel(const double* __restrict__ C, int n, int ldc, int ldct,
+                                       __bf16* __restrict__ hi, __bf16* __restrict__ lo) {
+  const std::size_t e = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<std::size_t>(ldc) * ldct) return;
+  const int i = static_cast<int>(e / ldct), nu = static_cast<int>(e % ldct);
+  const float x = nu < n ? static_cast<float>(C[static_cast<std::size_t>(nu) * ldc + i]) : 0.0f;
+  __bf16 h, l;
+  split_bf16(x, h, l);
+  hi[e] = h;
+  lo[e] = l;
+}
+
+template <int NI>
+void launch_u(Precision prec, const void* B, const OccupiedOperands& ops, int ldq, void* U, hipStream_t s) {
+  const int nqb = ldq / 128;
+  dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (16 * NI));
+  if (prec == Precision::FP64) {
+    hipLaunchKernelGGL((occupied_u_kernel<double, NI>), grid, dim3(256), 0, s, static_cast<const double*>(B),
+                       ops.c64, static_cast<double*>(U), ops.n, ldq, ops.ldc, ops.r, nqb);
+  } else if (prec == Precision::FP32) {
+    hipLaunchKernelGGL((occupied_u_kernel<float, NI>), grid, dim3(256), 0, s, static_cast<const float*>(B),
+                       ops.c32, static_cast<float*>(U), ops.n, ldq, ops.ldc, ops.r, nqb);
+  } else {
+    hipLaunchKernelGGL((occupied_u_bf16x3_kernel<NI>), grid, dim3(256), 0, s, static_cast<const float*>(B),
+                       static_cast<const __bf16*>(ops.ct_hi), static_cast<const __bf16*>(ops.ct_lo),
+                       static_cast<float*>(U), ops.n, ldq, ops.ldct, ops.r, nqb);
+  }
+  VQC_LAUNCH("occupied_u_kernel");
+}
+
+constexpr int kSyrkTargetBlocks = 2048;
+constexpr int kMaxSplits = 64;
+
+}  // namespace
+
+int occupied_tile(int r) {
+  int best = 1;
+  std::size_t best_pad = round_up(r, 16);
+  for (int ni = 1; ni <= 8; ++ni) {
+    const std::size_t padded = round_up(r, 16 * ni);
+    if (padded <= best_pad) {
+      best = ni;
+      best_pad = padded;
+    }
+  }
+  return best;
+}
+
+void pack_density(const double* D, int n, double* dpacked, hipStream_t s) {
+  const std::size_t np = pair_count(n);
+  hipLaunchKernelGGL(pack_density_kernel, dim3((np + 255) / 256), dim3(256), 0, s, D, n, dpacked);
+  VQC_LAUNCH("pack_density_kernel");
+}
+
+namespace {
+int rho_vector(Precision prec) { return prec == Precision::FP64 ? 2 : 4; }
+int rho_chunks(std::size_t npair, int ldq, Precision prec) {
+  const int cols = (ldq + kRhoThreads * rho_vector(prec) - 1) / (kRhoThreads * rho_vector(prec));
+  const int want = std::max(1, 2048 / cols);
+  return static_cast<int>(std::min<std::size_t>(want, std::max<std::size_t>(1, npair / 64)));
+}
+}  // namespace
+
+std::size_t coulomb_partial_size(std::size_t npair, int ldq) {
+  return static_cast<std::size_t>(std::max(rho_chunks(npair, ldq, Precision::FP64),
+                                           rho_chunks(npair, ldq, Precision::FP32))) *
+         ldq;
+}
+
+void coulomb_rho(Precision prec, const void* B, const double* dpacked, std::size_t npair, int ldq, double* rho,
+                 double* partial, hipStream_t s) {
+  const int v = rho_vector(prec);
+  const int chunks = rho_chunks(npair, ldq, prec);
+  const std::size_t chunk = (npair + chunks - 1) / chunks;
+  dim3 grid((ldq + kRhoThreads * v - 1) / (kRhoThreads * v), chunks);
+  if (prec == Precision::FP64)
+    hipLaunchKernelGGL((coulomb_rho_kernel<double, 2>), grid, dim3(kRhoThreads), 0, s,
+                       static_cast<const double*>(B), dpacked, npair, ldq, chunk, partial);
+  else
+    hipLaunchKernelGGL((coulomb_rho_kernel<float, 4>), grid, dim3(kRhoThreads), 0, s,
+                       static_cast<const float*>(B), dpacked, npair, ldq, chunk, partial);
+  VQC_LAUNCH("coulomb_rho_kernel");
+  hipLaunchKernelGGL(sum_rows_kernel, dim3((ldq + 255) / 256), dim3(256), 0, s, partial, chunks, ldq, rho);
+  VQC_LAUNCH("sum_rows_kernel");
+}
+
+void coulomb_matrix(Precision prec, const void* B, const double* rho, std::size_t npair, int ldq, int n, double* J,
+                    hipStream_t s) {
+  const std::size_t lds = static_cast<std::size_t>(ldq) * sizeof(double);
+  if (lds > 160 * 1024) throw NotImplemented("auxiliary dimension exceeds the LDS-resident rho limit");
+  const unsigned blocks = static_cast<unsigned>(std::min<std::size_t>((npair + 3) / 4, 4096));
+  if (prec == Precision::FP64)
+    hipLaunchKernelGGL((coulomb_matrix_kernel<double, 2>), dim3(blocks), dim3(256), lds, s,
+                       static_cast<const double*>(B), rho, npair, ldq, n, J);
+  else
+    hipLaunchKernelGGL((coulomb_matrix_kernel<float, 4>), dim3(blocks), dim3(256), lds, s,
+                       static_cast<const float*>(B), rho, npair, ldq, n, J);
+  VQC_LAUNCH("coulomb_matrix_kernel");
+}
+
+void prepare_occupied(Precision prec, OccupiedOperands& ops, hipStream_t s) {
+  const std::size_t count = static_cast<std::size_t>(ops.n) * ops.ldc;
+  if (prec == Precision::FP32) {
+    hipLaunchKernelGGL(convert_kernel, dim3(std::min<std::size_t>((count + 255) / 256, 4096)), dim3(256), 0, s,
+                       ops.c64, ops.c32, count);
+    VQC_LAUNCH("convert_kernel");
+  } else if (prec == Precision::BF16X3) {
+    const std::size_t e = static_cast<std::size_t>(ops.ldc) * ops.ldct;
+    hipLaunchKernelGGL(split_transpose_kernel, dim3((e + 255) / 256), dim3(256), 0, s, ops.c64, ops.n, ops.ldc,
+                       ops.ldct, static_cast<__bf16*>(ops.ct_hi), static_cast<__bf16*>(ops.ct_lo));
+    VQC_LAUNCH("split_transpose_kernel");
+  }
+}
+
+void occupied_u(Precision prec, const void* B, const OccupiedOperands& ops, int ldq, void* U, hipStream_t s) {
+  switch (ops.ni) {
+    case 1: launch_u<1>(prec, B, ops, ldq, U, s); break;
+    case 2: launch_u<2>(prec, B, ops, ldq, U, s); break;
+    case 3: launch_u<3>(prec, B, ops, ldq, U, s); break;
+    case 4: launch_u<4>(prec, B, ops, ldq, U, s); break;
+    case 5: launch_u<5>(prec, B, ops, ldq, U, s); break;
+    case 6: launch_u<6>(prec, B, ops, ldq, U, s); break;
+    case 7: launch_u<7>(prec, B, ops, ldq, U, s); break;
+    case 8: launch_u<8>(prec, B, ops, ldq, U, s); break;
+    default: throw std::logic_error("occupied tile width out of range");
+  }
+}
+
+std::size_t exchange_partial_size(int n) {
+  return static_cast<std::size_t>(kMaxSplits) * 2 * static_cast<std::size_t>(n) * n;
+}
+
+void exchange_syrk(Precision prec, const void* U, int n, int r, int r_pos, int ldq, double* K, double* partial,
+                   hipStream_t s) {
+  const int nb = (n + 63) / 64;
+  const int tiles = nb * (nb + 1) / 2;
+  const std::size_t kd = static_cast<std::size_t>(r) * ldq;
+  const std::size_t bk = prec == Precision::BF16X3 ? 32 : 16;
+  int slices = 0;
+  auto run = [&](std::size_t kb, std::size_t ke, double sign) {
+    if (ke <= kb) return;
+    const std::size_t len = ke - kb;
+    int splits = std::max(1, std::min(kMaxSplits, kSyrkTargetBlocks / tiles));
+    std::size_t chunk = round_up((len + splits - 1) / splits, bk);
+    splits = static_cast<int>((len + chunk - 1) / chunk);
+    dim3 grid(tiles, splits);
+    double* out = partial + static_cast<std::size_t>(slices) * n * n;
+    if (prec == Precision::FP64)
+      hipLaunchKernelGGL(exchange_syrk_kernel<double>, grid, dim3(256), 0, s, static_cast<const double*>(U), kd, n,
+                         kb, ke, chunk, sign, out);
+    else if (prec == Precision::FP32)
+      hipLaunchKernelGGL(exchange_syrk_kernel<float>, grid, dim3(256), 0, s, static_cast<const float*>(U), kd, n,
+                         kb, ke, chunk, sign, out);
+    else
+      hipLaunchKernelGGL(exchange_syrk_bf16x3_kernel, grid, dim3(256), 0, s, static_cast<const float*>(U), kd, n,
+                         kb, ke, chunk, sign, out);
+    VQC_LAUNCH("exchange_syrk_kernel");
+    slices += splits;
+  };
+  run(0, static_cast<std::size_t>(r_pos) * ldq, 1.0);
+  run(static_cast<std::size_t>(r_pos) * ldq, kd, -1.0);
+  if (slices == 0) {
+    VQC_HIP(hipMemsetAsync(K, 0, static_cast<std::size_t>(n) * n * sizeof(double), s));
+    return;
+  }
+  const std::size_t e = static_cast<std::size_t>(n) * n;
+  hipLaunchKernelGGL(reduce_exchange_kernel, dim3((e + 255) / 256), dim3(256), 0, s, partial, slices, n, K);
+  VQC_LAUNCH("reduce_exchange_kernel");
+}
+
+void convert_f64_to_f32(const double* in, float* out, std::size_t count, hipStream_t s) {
+  hipLaunchKernelGGL(convert_kernel, dim3(std::min<std::size_t>((count + 255) / 256, 65536)), dim3(256), 0, s, in,
+                     out, count);
+  VQC_LAUNCH("convert_kernel");
+}
+
+}  // namespace vqc

@@ -1,0 +1,58 @@
+// Density-fitted Coulomb and exchange contractions on gfx950.
+//
+// Storage (HBM): the fitted tensor B is kept once, packed over AO pairs,
+//   B[p][Q], p = mu(mu+1)/2 + nu (mu >= nu), row stride ldq = naux rounded up
+//   to a multiple of 128 (zero padding), in FP64 and, for reduced precision,
+//   also in FP32.  The occupied factor C is nbf x ldc (zero-padded columns).
+// J (HBM-bound): rho_Q = sum_p d_p B[p][Q] then J_p = sum_Q B[p][Q] rho_Q,
+//   two streaming passes over B, FP64 accumulation.
+// K (MFMA-bound): U[mu][i][Q] = sum_nu B[(mu,nu)][Q] C[nu][i] (one GEMM per mu,
+//   rows of B gathered from the packed layout), then K = W S W^T with
+//   W = U viewed as nbf x (r ldq) and S the sign of each factor column, as a
+//   split-K lower-triangular SYRK with a deterministic reduction.
+#pragma once
+
+#include "common.hpp"
+
+namespace vqc {
+
+enum class Precision : int { FP64 = 0, FP32 = 1, BF16X3 = 2 };
+
+inline std::size_t round_up(std::size_t x, std::size_t m) { return (x + m - 1) / m * m; }
+
+// Pick the i-tile width (16 * ni) for r columns; returns ni in 1..8.
+int occupied_tile(int r);
+
+// ---- J -------------------------------------------------------------------
+void pack_density(const double* D, int n, double* dpacked, hipStream_t s);
+// rho (ldq doubles) from packed density weights; `partial` needs
+// coulomb_partial_size(...) doubles.
+std::size_t coulomb_partial_size(std::size_t npair, int ldq);
+void coulomb_rho(Precision prec, const void* B, const double* dpacked, std::size_t npair, int ldq, double* rho,
+                 double* partial, hipStream_t s);
+void coulomb_matrix(Precision prec, const void* B, const double* rho, std::size_t npair, int ldq, int n, double* J,
+                    hipStream_t s);
+
+// ---- K -------------------------------------------------------------------
+// C64: n x ldc FP64 factor.  Prepares the precision-specific operand copies.
+struct OccupiedOperands {
+  int n = 0, r = 0, ldc = 0, ni = 1, ldct = 0;
+  const double* c64 = nullptr;
+  float* c32 = nullptr;       // n x ldc
+  void* ct_hi = nullptr;      // ldc x ldct bf16 (transposed), hi part
+  void* ct_lo = nullptr;      // lo part
+};
+void prepare_occupied(Precision prec, OccupiedOperands& ops, hipStream_t s);
+// U[mu][i][Q] (r x ldq per mu) in FP64 (fp64) or FP32 (fp32/bf16x3).
+void occupied_u(Precision prec, const void* B, const OccupiedOperands& ops, int ldq, void* U, hipStream_t s);
+// K (n x n, FP64, both triangles) = sum_k sign_k W[mu][k] W[lam][k] where
+// columns i < r_pos carry +1 and the rest -1.  `partial` needs
+// exchange_partial_size(n) doubles.
+std::size_t exchange_partial_size(int n);
+void exchange_syrk(Precision prec, const void* U, int n, int r, int r_pos, int ldq, double* K, double* partial,
+                   hipStream_t s);
+
+// ---- conversions ---------------------------------------------------------
+void convert_f64_to_f32(const double* in, float* out, std::size_t count, hipStream_t s);
+
+}  // namespace vqc

@@ -1,0 +1,593 @@
+// DfPlan: setup (integrals, metric factor, fitted tensor), Fock builds and the
+// device-resident RHF SCF.
+//
+// Setup follows the reference's prepared CUDA plan
+// (src/scf/fock_prepared.cpp:216-330, src/scf/cuda/df_plan_setup.cpp:456-830)
+// with an MI355X-specific factorisation: the (P|Q) metric is Cholesky-factored
+// in FP64 (rocSOLVER) and B = A L^-T is formed in place by a triangular solve.
+// At full rank this gives the same J and K as the reference's M^-1/2 (B B^T =
+// A M^-1 A^T); a metric whose smallest Cholesky pivot falls below
+// 10 x threshold x max(M_ii) takes the reference's eigen-cutoff route
+// (src/integrals/density_fitting_metric.cpp:101-152) instead.
+//
+// The SCF mirrors run_rhf_host_plan (src/scf/solver/mean_field_driver.cpp:
+// 233-345) but keeps every matrix on the device: per iteration only the
+// energy and the density RMS (two doubles) cross to the host.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <numeric>
+#include <stdexcept>
+
+#include "scf.hpp"
+
+namespace vqc {
+namespace {
+
+constexpr std::size_t kTrsmColumns = 1 << 16;
+
+struct Event {
+  hipEvent_t e = nullptr;
+  Event() { VQC_HIP(hipEventCreate(&e)); }
+  ~Event() {
+    if (e) (void)hipEventDestroy(e);
+  }
+  Event(const Event&) = delete;
+  Event& operator=(const Event&) = delete;
+  Event(Event&& o) noexcept : e(std::exchange(o.e, nullptr)) {}
+};
+
+void check_basis_limits(const System& sys, int lmax, const char* what) {
+  for (const auto& s : sys.shells)
+    if (static_cast<int>(s.l) > lmax)
+      throw NotImplemented(std::string(what) + " shells above l=" + std::to_string(lmax) +
+                           " are not supported by the device kernels");
+}
+
+}  // namespace
+
+DfPlan::DfPlan(int device, const System& orbital, const System& auxiliary, const PlanOptions& options)
+    : device_(device), orb_(orbital), opts_(options) {
+  VQC_HIP(hipSetDevice(device_));
+  VQC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  VQC_BLAS(rocblas_create_handle(&blas_));
+  VQC_BLAS(rocblas_set_stream(blas_, stream_));
+  try {
+    setup(auxiliary);
+  } catch (...) {
+    rocblas_destroy_handle(blas_);
+    (void)hipStreamDestroy(stream_);
+    throw;
+  }
+}
+
+DfPlan::~DfPlan() {
+  (void)hipSetDevice(device_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  if (blas_) rocblas_destroy_handle(blas_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void DfPlan::sync() { VQC_HIP(hipStreamSynchronize(stream_)); }
+
+std::size_t DfPlan::device_bytes() const {
+  return boys_.bytes() + S_.bytes() + H_.bytes() + B64_.bytes() + B32_.bytes() + dpacked_.bytes() +
+         rho_.bytes() + rho_partial_.bytes() + J_.bytes() + K_.bytes() + syrk_partial_.bytes() + D_.bytes() +
+         c64_.bytes() + c32_.bytes() + ct_hi_.bytes() + ct_lo_.bytes() + U_.bytes();
+}
+
+void DfPlan::setup(const System& aux) {
+  check_basis_limits(orb_, kMaxOrbitalL, "orbital");
+  check_basis_limits(aux, kMaxAuxiliaryL, "auxiliary");
+  n_ = static_cast<int>(orb_.nbf());
+  naux_ = static_cast<int>(aux.nbf());
+  if (n_ <= 0 || naux_ <= 0) throw std::invalid_argument("empty orbital or auxiliary basis");
+  ldq_ = static_cast<int>(round_up(naux_, 128));
+  npair_ = pair_count(n_);
+  enuc_ = orb_.nuclear_repulsion();
+  nocc_ = orb_.electrons / 2;
+  flag_.resize(1);
+
+  boys_.upload(boys_table_host(), stream_);
+  DeviceBasis ob;
+  upload_basis(orb_, ob, false, stream_);
+  AuxiliaryLayout al;
+  build_auxiliary_layout(aux, al, stream_);
+
+  const std::size_t nn = static_cast<std::size_t>(n_) * n_;
+  S_.resize(nn);
+  H_.resize(nn);
+  {
+    DeviceArray<double> T(nn), V(nn);
+    compute_one_electron(orb_, ob, boys_.data(), S_.data(), T.data(), V.data(), stream_);
+    assemble_fock(T.data(), V.data(), V.data(), 1.0, 0.0, nn, H_.data(), stream_);
+    sync();
+  }
+  DeviceArray<double> M(static_cast<std::size_t>(naux_) * naux_);
+  compute_metric(al, boys_.data(), M.data(), stream_);
+  B64_.resize(npair_ * ldq_);
+  compute_three_center(orb_, ob, al, boys_.data(), B64_.data(), ldq_, opts_.screening * 1e-10, stream_);
+  factor_metric(M, B64_);
+  VQC_HIP(hipMemsetAsync(flag_.data(), 0, sizeof(int), stream_));
+  check_finite(B64_.data(), B64_.size(), flag_.data(), stream_);
+  int bad = 0;
+  flag_.download(&bad, 1, stream_);
+  sync();
+  if (bad) throw NumericalFailure("non-finite fitted three-centre tensor");
+}
+
+void DfPlan::factor_metric(DeviceArray<double>& M, DeviceArray<double>& A) {
+  const int na = naux_;
+  const std::size_t mm = static_cast<std::size_t>(na) * na;
+  std::vector<double> mdiag(na), ldiag(na);
+  DeviceArray<double> diag(na);
+  extract_diagonal(M.data(), na, diag.data(), stream_);
+  diag.download(mdiag.data(), na, stream_);
+  DeviceArray<double> L(mm);
+  VQC_HIP(hipMemcpyAsync(L.data(), M.data(), mm * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  DeviceArray<rocblas_int> info(1);
+  VQC_BLAS(rocsolver_dpotrf(blas_, rocblas_fill_lower, na, L.data(), na, info.data()));
+  rocblas_int hinfo = 0;
+  info.download(&hinfo, 1, stream_);
+  extract_diagonal(L.data(), na, diag.data(), stream_);
+  diag.download(ldiag.data(), na, stream_);
+  sync();
+  const double maxdiag = *std::max_element(mdiag.begin(), mdiag.end());
+  if (!(maxdiag > 0.0) || !std::isfinite(maxdiag)) throw NumericalFailure("Coulomb metric has no positive diagonal");
+  double minpiv = std::numeric_limits<double>::infinity(), maxpiv = 0.0;
+  for (double d : ldiag) {
+    minpiv = std::min(minpiv, d * d);
+    maxpiv = std::max(maxpiv, d * d);
+  }
+  const bool use_cholesky = hinfo == 0 && std::isfinite(minpiv) && minpiv > 10.0 * opts_.metric_threshold * maxdiag;
+  const double one = 1.0, zero = 0.0;
+  if (use_cholesky) {
+    cholesky_ = true;
+    rank_ = na;
+    condition_ = maxpiv / minpiv;
+    for (std::size_t c0 = 0; c0 < npair_; c0 += kTrsmColumns) {
+      const int cols = static_cast<int>(std::min(kTrsmColumns, npair_ - c0));
+      VQC_BLAS(rocblas_dtrsm(blas_, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                             rocblas_diagonal_non_unit, na, cols, &one, L.data(), na, A.data() + c0 * ldq_, ldq_));
+    }
+    sync();
+    return;
+  }
+  // Eigen-cutoff route: X = V diag(lambda^-1/2 | lambda > thr lambda_max) V^T, B = A X.
+  cholesky_ = false;
+  DeviceArray<double> V(mm), w(na), e(na);
+  VQC_HIP(hipMemcpyAsync(V.data(), M.data(), mm * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  VQC_BLAS(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_lower, na, V.data(), na, w.data(), e.data(),
+                            info.data()));
+  std::vector<double> hw(na);
+  w.download(hw.data(), na, stream_);
+  info.download(&hinfo, 1, stream_);
+  sync();
+  if (hinfo != 0) throw NumericalFailure("metric eigensolver did not converge");
+  const double largest = hw.back();
+  if (!(largest > 0.0)) throw NumericalFailure("Coulomb metric has no positive eigenspace");
+  const double cutoff = opts_.metric_threshold * largest;
+  std::vector<double> scale(na, 0.0);
+  rank_ = 0;
+  double smallest = largest;
+  for (int k = 0; k < na; ++k)
+    if (hw[k] > cutoff) {
+      scale[k] = 1.0 / std::sqrt(hw[k]);
+      ++rank_;
+      smallest = std::min(smallest, hw[k]);
+    }
+  if (rank_ == 0) throw NumericalFailure("metric threshold removed every auxiliary direction");
+  condition_ = largest / smallest;
+  // Scale columns: Vs[:, k] = V[:, k] * scale[k]  (column-major, scale applied as w^-1/2 of 1/scale^2).
+  std::vector<double> inv(na);
+  for (int k = 0; k < na; ++k) inv[k] = scale[k] > 0.0 ? 1.0 / (scale[k] * scale[k]) : std::numeric_limits<double>::infinity();
+  DeviceArray<double> winv(na), Vs(mm), X(mm);
+  winv.upload(inv, stream_);
+  scale_columns_inv_sqrt(V.data(), winv.data(), na, Vs.data(), stream_);
+  VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_transpose, na, na, na, &one, Vs.data(), na,
+                         V.data(), na, &zero, X.data(), na));
+  DeviceArray<double> tmp(kTrsmColumns * ldq_);
+  VQC_HIP(hipMemsetAsync(tmp.data(), 0, tmp.bytes(), stream_));
+  for (std::size_t c0 = 0; c0 < npair_; c0 += kTrsmColumns) {
+    const int cols = static_cast<int>(std::min(kTrsmColumns, npair_ - c0));
+    VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, na, cols, na, &one, X.data(), na,
+                           A.data() + c0 * ldq_, ldq_, &zero, tmp.data(), ldq_));
+    VQC_HIP(hipMemcpyAsync(A.data() + c0 * ldq_, tmp.data(), static_cast<std::size_t>(cols) * ldq_ * sizeof(double),
+                           hipMemcpyDeviceToDevice, stream_));
+  }
+  sync();
+}
+
+void DfPlan::set_precision(Precision p) {
+  prec_ = p;
+  ensure_reduced();
+}
+
+void DfPlan::ensure_reduced() {
+  if (prec_ == Precision::FP64 || B32_.size()) return;
+  B32_.resize(npair_ * ldq_);
+  convert_f64_to_f32(B64_.data(), B32_.data(), B32_.size(), stream_);
+  sync();
+}
+
+void DfPlan::ensure_factor(int r) {
+  const std::size_t nn = static_cast<std::size_t>(n_) * n_;
+  if (!J_.size()) {
+    J_.resize(nn);
+    K_.resize(nn);
+    D_.resize(nn);
+    dpacked_.resize(npair_);
+    rho_.resize(ldq_);
+    rho_partial_.resize(coulomb_partial_size(npair_, ldq_));
+    syrk_partial_.resize(exchange_partial_size(n_));
+  }
+  r = std::max(r, 1);
+  const int ni = occupied_tile(r);
+  const int ldc = static_cast<int>(round_up(r, 16 * ni));
+  if (r <= factor_cap_ && ni == ni_ && ldc == ldc_) return;
+  ni_ = ni;
+  ldc_ = ldc;
+  factor_cap_ = r;
+  const int ldct = static_cast<int>(round_up(n_, 32));
+  c64_.resize(static_cast<std::size_t>(n_) * ldc_);
+  c32_.resize(static_cast<std::size_t>(n_) * ldc_);
+  ct_hi_.resize(static_cast<std::size_t>(ldc_) * ldct);
+  ct_lo_.resize(static_cast<std::size_t>(ldc_) * ldct);
+  U_.release();
+  U_.resize(static_cast<std::size_t>(n_) * r * ldq_ * sizeof(double));
+  ops_ = OccupiedOperands{};
+  ops_.n = n_;
+  ops_.ldc = ldc_;
+  ops_.ni = ni_;
+  ops_.ldct = ldct;
+  ops_.c64 = c64_.data();
+  ops_.c32 = c32_.data();
+  ops_.ct_hi = ct_hi_.data();
+  ops_.ct_lo = ct_lo_.data();
+}
+
+void DfPlan::build_jk(Precision p, const double* D, int r, int r_pos) {
+  if (p != Precision::FP64) ensure_reduced();
+  const void* B = p == Precision::FP64 ? static_cast<const void*>(B64_.data()) : static_cast<const void*>(B32_.data());
+  if (opts_.exchange) {
+    ops_.r = r;
+    if (r > 0) {
+      prepare_occupied(p, ops_, stream_);
+      occupied_u(p, B, ops_, ldq_, U_.data(), stream_);
+    }
+    exchange_syrk(p, U_.data(), n_, r, r_pos, ldq_, K_.data(), syrk_partial_.data(), stream_);
+  }
+  if (opts_.coulomb) {
+    pack_density(D, n_, dpacked_.data(), stream_);
+    coulomb_rho(p, B, dpacked_.data(), npair_, ldq_, rho_.data(), rho_partial_.data(), stream_);
+    coulomb_matrix(p, B, rho_.data(), npair_, ldq_, n_, J_.data(), stream_);
+  }
+}
+
+void DfPlan::upload_factor(const std::vector<double>& occupied, int nocc) {
+  ensure_factor(nocc);
+  std::vector<double> c(static_cast<std::size_t>(n_) * ldc_, 0.0);
+  const double root2 = std::sqrt(2.0);
+  for (int nu = 0; nu < n_; ++nu)
+    for (int i = 0; i < nocc; ++i)
+      c[static_cast<std::size_t>(nu) * ldc_ + i] = root2 * occupied[static_cast<std::size_t>(nu) * nocc + i];
+  c64_.upload(c.data(), c.size(), stream_);
+  // D = 2 C C^T = Cs Cs^T with Cs = sqrt(2) C; c64_ is the column-major ldc x n matrix Cs^T.
+  const double one = 1.0, zero = 0.0;
+  VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, n_, n_, std::max(nocc, 1), &one,
+                         c64_.data(), ldc_, c64_.data(), ldc_, &zero, D_.data(), n_));
+}
+
+int DfPlan::factorise_density(const double* D) {
+  // D = V diag(w) V^T; factor columns sqrt|w_k| v_k, positive eigenvalues first.
+  const std::size_t nn = static_cast<std::size_t>(n_) * n_;
+  DeviceArray<double> V(nn), w(n_), e(n_);
+  DeviceArray<rocblas_int> info(1);
+  VQC_HIP(hipMemcpyAsync(V.data(), D, nn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  VQC_BLAS(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_lower, n_, V.data(), n_, w.data(), e.data(),
+                            info.data()));
+  std::vector<double> hw(n_);
+  rocblas_int hinfo = 0;
+  w.download(hw.data(), n_, stream_);
+  info.download(&hinfo, 1, stream_);
+  sync();
+  if (hinfo != 0) throw NumericalFailure("density eigensolver did not converge");
+  double largest = 0.0;
+  for (double x : hw) largest = std::max(largest, std::abs(x));
+  std::vector<int> pos, neg;
+  for (int k = n_ - 1; k >= 0; --k)
+    if (hw[k] > 1e-14 * largest) pos.push_back(k);
+  for (int k = 0; k < n_; ++k)
+    if (hw[k] < -1e-14 * largest) neg.push_back(k);
+  std::vector<int> idx = pos;
+  idx.insert(idx.end(), neg.begin(), neg.end());
+  const int r = static_cast<int>(idx.size());
+  ensure_factor(r);
+  DeviceArray<int> didx;
+  if (r) didx.upload(idx, stream_);
+  factor_from_eigen(V.data(), w.data(), r ? didx.data() : nullptr, n_, r, ldc_, c64_.data(), stream_);
+  sync();
+  return static_cast<int>(pos.size()) | (r << 16);
+}
+
+JkResult DfPlan::evaluate(const std::vector<double>& density) {
+  VQC_HIP(hipSetDevice(device_));
+  const std::size_t nn = static_cast<std::size_t>(n_) * n_;
+  if (density.size() != nn) throw std::invalid_argument("density shape mismatch");
+  double scale = 0.0, asym = 0.0;
+  for (int i = 0; i < n_; ++i)
+    for (int j = 0; j < n_; ++j) {
+      const double a = density[static_cast<std::size_t>(i) * n_ + j], b = density[static_cast<std::size_t>(j) * n_ + i];
+      if (!std::isfinite(a)) throw std::invalid_argument("density must be finite");
+      scale = std::max(scale, std::abs(a));
+      asym = std::max(asym, std::abs(a - b));
+    }
+  if (asym > 1e-10 * std::max(1.0, scale))
+    throw std::invalid_argument("density-fitted exchange requires a symmetric density");
+  std::vector<double> sym(nn);
+  for (int i = 0; i < n_; ++i)
+    for (int j = 0; j < n_; ++j)
+      sym[static_cast<std::size_t>(i) * n_ + j] =
+          0.5 * (density[static_cast<std::size_t>(i) * n_ + j] + density[static_cast<std::size_t>(j) * n_ + i]);
+  ensure_factor(1);
+  D_.upload(sym.data(), nn, stream_);
+  int packed = opts_.exchange ? factorise_density(D_.data()) : 0;
+  const int r_pos = packed & 0xffff, r = packed >> 16;
+  build_jk(prec_, D_.data(), r, r_pos);
+  JkResult out;
+  out.J.resize(nn);
+  out.K.resize(nn);
+  J_.download(out.J.data(), nn, stream_);
+  K_.download(out.K.data(), nn, stream_);
+  sync();
+  if (!opts_.coulomb) std::fill(out.J.begin(), out.J.end(), 0.0);
+  if (!opts_.exchange) std::fill(out.K.begin(), out.K.end(), 0.0);
+  return out;
+}
+
+JkResult DfPlan::evaluate_occupied(const std::vector<double>& occupied, int nocc) {
+  VQC_HIP(hipSetDevice(device_));
+  if (nocc < 0 || occupied.size() != static_cast<std::size_t>(n_) * nocc)
+    throw std::invalid_argument("occupied factor shape mismatch");
+  for (double x : occupied)
+    if (!std::isfinite(x)) throw std::invalid_argument("occupied factor must be finite");
+  ensure_factor(nocc);
+  upload_factor(occupied, nocc);
+  build_jk(prec_, D_.data(), nocc, nocc);
+  const std::size_t nn = static_cast<std::size_t>(n_) * n_;
+  JkResult out;
+  out.J.resize(nn);
+  out.K.resize(nn);
+  J_.download(out.J.data(), nn, stream_);
+  K_.download(out.K.data(), nn, stream_);
+  sync();
+  return out;
+}
+
+std::array<double, 4> DfPlan::benchmark_occupied(const std::vector<double>& occupied, int nocc, unsigned repeat) {
+  VQC_HIP(hipSetDevice(device_));
+  if (repeat == 0) throw std::invalid_argument("repeat must be positive");
+  if (nocc <= 0 || occupied.size() != static_cast<std::size_t>(n_) * nocc)
+    throw std::invalid_argument("occupied factor shape mismatch");
+  ensure_factor(nocc);
+  upload_factor(occupied, nocc);
+  const Precision p = prec_;
+  if (p != Precision::FP64) ensure_reduced();
+  build_jk(p, D_.data(), nocc, nocc);  // warm-up
+  sync();
+  const void* B = p == Precision::FP64 ? static_cast<const void*>(B64_.data()) : static_cast<const void*>(B32_.data());
+  std::vector<Event> ev(static_cast<std::size_t>(repeat) * 4);
+  for (unsigned it = 0; it < repeat; ++it) {
+    VQC_HIP(hipEventRecord(ev[it * 4 + 0].e, stream_));
+    ops_.r = nocc;
+    prepare_occupied(p, ops_, stream_);
+    occupied_u(p, B, ops_, ldq_, U_.data(), stream_);
+    VQC_HIP(hipEventRecord(ev[it * 4 + 1].e, stream_));
+    exchange_syrk(p, U_.data(), n_, nocc, nocc, ldq_, K_.data(), syrk_partial_.data(), stream_);
+    VQC_HIP(hipEventRecord(ev[it * 4 + 2].e, stream_));
+    pack_density(D_.data(), n_, dpacked_.data(), stream_);
+    coulomb_rho(p, B, dpacked_.data(), npair_, ldq_, rho_.data(), rho_partial_.data(), stream_);
+    coulomb_matrix(p, B, rho_.data(), npair_, ldq_, n_, J_.data(), stream_);
+    VQC_HIP(hipEventRecord(ev[it * 4 + 3].e, stream_));
+  }
+  sync();
+  std::array<double, 4> ms{0.0, 0.0, 0.0, 0.0};
+  for (unsigned it = 0; it < repeat; ++it) {
+    float a = 0, b = 0, c = 0, d = 0;
+    VQC_HIP(hipEventElapsedTime(&a, ev[it * 4 + 0].e, ev[it * 4 + 1].e));
+    VQC_HIP(hipEventElapsedTime(&b, ev[it * 4 + 1].e, ev[it * 4 + 2].e));
+    VQC_HIP(hipEventElapsedTime(&c, ev[it * 4 + 2].e, ev[it * 4 + 3].e));
+    VQC_HIP(hipEventElapsedTime(&d, ev[it * 4 + 0].e, ev[it * 4 + 3].e));
+    ms[0] += a;
+    ms[1] += b;
+    ms[2] += c;
+    ms[3] += d;
+  }
+  for (auto& x : ms) x /= repeat;
+  return ms;
+}
+
+std::vector<double> DfPlan::hcore() const {
+  std::vector<double> h(static_cast<std::size_t>(n_) * n_);
+  VQC_HIP(hipSetDevice(device_));
+  H_.download(h.data(), h.size(), stream_);
+  VQC_HIP(hipStreamSynchronize(stream_));
+  return h;
+}
+
+std::vector<double> DfPlan::overlap() const {
+  std::vector<double> s(static_cast<std::size_t>(n_) * n_);
+  VQC_HIP(hipSetDevice(device_));
+  S_.download(s.data(), s.size(), stream_);
+  VQC_HIP(hipStreamSynchronize(stream_));
+  return s;
+}
+
+ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial_density) {
+  VQC_HIP(hipSetDevice(device_));
+  if (orb_.multiplicity != 1 || orb_.electrons % 2 != 0)
+    throw NotImplemented("only closed-shell restricted SCF is implemented");
+  if (!opts_.coulomb || !opts_.exchange || opts_.cj != 1.0 || opts_.ck != -0.5)
+    throw NotImplemented("solve requires the standard Hartree-Fock J/K coefficients");
+  const int n = n_, nocc = nocc_;
+  if (nocc > n) throw std::runtime_error("basis has fewer orbitals than occupied electron pairs");
+  const std::size_t nn = static_cast<std::size_t>(n) * n;
+  const double one = 1.0, zero = 0.0;
+  ensure_factor(nocc);
+
+  DeviceArray<double> X(nn), V(nn), W(nn), w(n), e(n), T1(nn), T2(nn), F(nn), Fe(nn), Dn(nn);
+  DeviceArray<rocblas_int> info(1);
+  DeviceArray<double> scalars(4);
+  auto eigh = [&](const double* A, double* vectors, double* values) {
+    VQC_HIP(hipMemcpyAsync(vectors, A, nn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    VQC_BLAS(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_lower, n, vectors, n, values, e.data(),
+                              info.data()));
+  };
+  auto gemm = [&](rocblas_operation ta, rocblas_operation tb, int k, const double* A, int lda, const double* B,
+                  int ldb, double* C) {
+    VQC_BLAS(rocblas_dgemm(blas_, ta, tb, n, n, k, &one, A, lda, B, ldb, &zero, C, n));
+  };
+  // X = S^-1/2 (src/scf/reference/linalg.cpp:123-142).
+  eigh(S_.data(), V.data(), w.data());
+  {
+    std::vector<double> hw(n);
+    w.download(hw.data(), n, stream_);
+    sync();
+    if (hw.front() < 1e-10) throw NumericalFailure("overlap matrix is singular or severely linearly dependent");
+  }
+  scale_columns_inv_sqrt(V.data(), w.data(), n, W.data(), stream_);
+  gemm(rocblas_operation_none, rocblas_operation_transpose, n, W.data(), n, V.data(), n, X.data());
+  // Orbitals of F in the X basis: C = X eig(X F X); new factor and density.
+  auto new_density = [&](const double* Fm, double* Dout) {
+    gemm(rocblas_operation_none, rocblas_operation_none, n, Fm, n, X.data(), n, T1.data());
+    gemm(rocblas_operation_none, rocblas_operation_none, n, X.data(), n, T1.data(), n, T2.data());
+    eigh(T2.data(), V.data(), w.data());
+    gemm(rocblas_operation_none, rocblas_operation_none, n, X.data(), n, V.data(), n, W.data());
+    occupied_from_orbitals(W.data(), n, nocc, ldc_, std::sqrt(2.0), c64_.data(), stream_);
+    VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, n, n, std::max(nocc, 1), &one,
+                           c64_.data(), ldc_, c64_.data(), ldc_, &zero, Dout, n));
+  };
+  ScfOutcome out;
+  // Initial density: core Hamiltonian guess (src/scf/initial_guess/density.cpp:635-672) or caller seed.
+  if (initial_density) {
+    D_.upload(initial_density->data(), nn, stream_);
+    // The seed's factor: eigen-decompose D (positive part) so K can use the occupied path.
+    const int packed = factorise_density(D_.data());
+    if ((packed >> 16) != (packed & 0xffff)) throw std::invalid_argument("initial density must be positive semidefinite");
+  } else {
+    new_density(H_.data(), D_.data());
+  }
+  int rank_now = initial_density ? -1 : nocc;
+  auto current_rank = [&]() { return rank_now; };
+
+  const unsigned cap = std::max(1u, o.diis_history);
+  DeviceArray<double> focks(nn * cap), errors(nn * cap), gram(static_cast<std::size_t>(cap) * cap), coef(cap);
+  DeviceArray<int> slots(cap);
+  std::vector<int> order;  // slots, oldest first
+  int next_slot = 0;
+
+  const bool reduced = prec_ != Precision::FP64;
+  const double e_switch = prec_ == Precision::BF16X3 ? 1e-6 : 1e-8;
+  const double d_switch = prec_ == Precision::BF16X3 ? 1e-5 : 1e-7;
+  bool in_reduced = reduced;
+  double previous = std::numeric_limits<double>::infinity();
+  bool converged = false;
+  unsigned it = 0;
+  double energy = 0.0, de = 0.0, rms = 0.0;
+  for (it = 1; it <= o.max_iterations; ++it) {
+    const Precision p = in_reduced ? prec_ : Precision::FP64;
+    int r = nocc, rp = nocc;
+    if (current_rank() < 0) {
+      const int packed = factorise_density(D_.data());
+      r = packed >> 16;
+      rp = packed & 0xffff;
+      rank_now = -1;
+    }
+    build_jk(p, D_.data(), r, rp);
+    ++out.fock_builds;
+    if (in_reduced) ++out.reduced_iterations;
+    assemble_fock(H_.data(), J_.data(), K_.data(), opts_.cj, opts_.ck, nn, F.data(), stream_);
+    dot_pair(D_.data(), H_.data(), F.data(), nn, scalars.data(), stream_);
+    // Residual FDS - SDF.
+    gemm(rocblas_operation_none, rocblas_operation_none, n, F.data(), n, D_.data(), n, T1.data());
+    gemm(rocblas_operation_none, rocblas_operation_none, n, T1.data(), n, S_.data(), n, T2.data());
+    // DIIS history (src/scf/solver/diis.cpp:17-46): push, Gram row, solve, extrapolate.
+    const double* Fuse = F.data();
+    if (cap >= 2) {
+      int slot;
+      if (order.size() < cap) {
+        slot = next_slot++;
+      } else {
+        slot = order.front();
+        order.erase(order.begin());
+      }
+      order.push_back(slot);
+      VQC_HIP(hipMemcpyAsync(focks.data() + static_cast<std::size_t>(slot) * nn, F.data(), nn * sizeof(double),
+                             hipMemcpyDeviceToDevice, stream_));
+      antisymmetrise(T2.data(), n, errors.data() + static_cast<std::size_t>(slot) * nn, stream_);
+      const int m = static_cast<int>(order.size());
+      slots.upload(order.data(), m, stream_);
+      diis_gram_row(errors.data(), nn, slot, m, slots.data(), static_cast<int>(cap), gram.data(), stream_);
+      if (m >= 2) {
+        diis_solve(gram.data(), m, slots.data(), static_cast<int>(cap), coef.data(), stream_);
+        diis_extrapolate(focks.data(), nn, m, slots.data(), coef.data(), Fe.data(), stream_);
+        Fuse = Fe.data();
+      }
+    }
+    new_density(Fuse, Dn.data());
+    rank_now = nocc;
+    rms_difference(Dn.data(), D_.data(), nn, scalars.data() + 2, stream_);
+    double hs[4];
+    scalars.download(hs, 4, stream_);
+    sync();
+    energy = 0.5 * (hs[0] + hs[1]) + enuc_;
+    rms = hs[2];
+    if (!std::isfinite(energy) || !std::isfinite(rms)) throw NumericalFailure("non-finite SCF iterate");
+    de = std::isfinite(previous) ? std::abs(energy - previous) : std::numeric_limits<double>::infinity();
+    std::swap(D_, Dn);
+    if (in_reduced) {
+      const bool settled = it > 1 && de < std::max(e_switch, o.energy_tolerance) &&
+                           rms < std::max(d_switch, o.density_tolerance);
+      if (settled || it >= o.max_iterations / 2) {
+        // Refinement: continue in FP64 with a fresh DIIS history.
+        in_reduced = false;
+        order.clear();
+        next_slot = 0;
+        previous = std::numeric_limits<double>::infinity();
+        continue;
+      }
+    } else if (it > 1 && std::isfinite(previous) && de < o.energy_tolerance && rms < o.density_tolerance) {
+      converged = true;
+      break;
+    }
+    previous = energy;
+  }
+  out.iterations = std::min(it, o.max_iterations);
+  out.energy = energy;
+  out.energy_change = de;
+  out.density_rms = rms;
+  if (!converged) {
+    out.converged = false;
+    return out;
+  }
+  // Finalisation (mean_field_driver.cpp:181-199): rebuild, diagonalise, rebuild.
+  build_jk(Precision::FP64, D_.data(), nocc, nocc);
+  assemble_fock(H_.data(), J_.data(), K_.data(), opts_.cj, opts_.ck, nn, F.data(), stream_);
+  new_density(F.data(), D_.data());
+  build_jk(Precision::FP64, D_.data(), nocc, nocc);
+  assemble_fock(H_.data(), J_.data(), K_.data(), opts_.cj, opts_.ck, nn, F.data(), stream_);
+  dot_pair(D_.data(), H_.data(), F.data(), nn, scalars.data(), stream_);
+  out.fock_builds += 2;
+  double hs[2];
+  scalars.download(hs, 2, stream_);
+  out.density.resize(nn);
+  D_.download(out.density.data(), nn, stream_);
+  sync();
+  out.energy = 0.5 * (hs[0] + hs[1]) + enuc_;
+  out.converged = std::isfinite(out.energy);
+  return out;
+}
+
+}  // namespace vqc

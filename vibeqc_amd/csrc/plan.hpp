@@ -1,0 +1,117 @@
+// Prepared DF-JK plan: owns the fitted tensor and the one-electron matrices on
+// one HIP device and runs Fock builds and the RHF SCF there.
+#pragma once
+
+#include <array>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "basis.hpp"
+#include "common.hpp"
+#include "integrals.hpp"
+#include "jk.hpp"
+
+namespace vqc {
+
+struct PlanOptions {
+  double screening = 1e-12;
+  double metric_threshold = 1e-10;
+  double cj = 1.0, ck = -0.5;
+  bool coulomb = true, exchange = true;
+};
+
+struct ScfOptions {
+  unsigned max_iterations = 100;
+  unsigned diis_history = 8;
+  double energy_tolerance = 1e-10;
+  double density_tolerance = 1e-8;
+};
+
+struct ScfOutcome {
+  bool converged = false;
+  double energy = 0.0, energy_change = 0.0, density_rms = 0.0;
+  unsigned iterations = 0;
+  unsigned reduced_iterations = 0;  // iterations run at reduced precision
+  uint64_t fock_builds = 0;
+  std::vector<double> density;
+};
+
+struct JkResult {
+  std::vector<double> J, K;
+};
+
+class DfPlan {
+ public:
+  DfPlan(int device, const System& orbital, const System& auxiliary, const PlanOptions& options);
+  ~DfPlan();
+  DfPlan(const DfPlan&) = delete;
+  DfPlan& operator=(const DfPlan&) = delete;
+
+  int nbf() const { return n_; }
+  int naux() const { return naux_; }
+  std::size_t npair() const { return npair_; }
+  int rank() const { return rank_; }
+  bool cholesky() const { return cholesky_; }
+  double condition() const { return condition_; }
+  double nuclear_repulsion() const { return enuc_; }
+  std::size_t device_bytes() const;
+  const System& system() const { return orb_; }
+  const PlanOptions& options() const { return opts_; }
+  void set_precision(Precision p);
+  Precision precision() const { return prec_; }
+
+  // One-electron matrices (host copies).
+  std::vector<double> hcore() const;
+  std::vector<double> overlap() const;
+
+  // Raw J/K for a symmetric density (host in/out).
+  JkResult evaluate(const std::vector<double>& density);
+  // Raw J/K for D = 2 C C^T (C row-major n x nocc, host).
+  JkResult evaluate_occupied(const std::vector<double>& occupied, int nocc);
+  // Timing harness (see vibeqc_fock_plan_benchmark_occupied).
+  std::array<double, 4> benchmark_occupied(const std::vector<double>& occupied, int nocc, unsigned repeat);
+
+  ScfOutcome solve(const ScfOptions& options, const std::vector<double>* initial_density);
+
+ private:
+  void setup(const System& aux);
+  void factor_metric(DeviceArray<double>& M, DeviceArray<double>& A);
+  void ensure_reduced();
+  void ensure_factor(int r);
+  // J/K on the device: D (n x n) and the factor already in c64_ with r
+  // columns, r_pos of them positive.  Results in J_, K_.
+  void build_jk(Precision p, const double* D, int r, int r_pos);
+  void upload_factor(const std::vector<double>& occupied, int nocc);
+  int factorise_density(const double* D);
+  void sync();
+
+  int device_;
+  hipStream_t stream_ = nullptr;
+  rocblas_handle blas_ = nullptr;
+  System orb_;
+  PlanOptions opts_;
+  Precision prec_ = Precision::FP64;
+  int n_ = 0, naux_ = 0, ldq_ = 0, nocc_ = 0;
+  std::size_t npair_ = 0;
+  int rank_ = 0;
+  bool cholesky_ = true;
+  double condition_ = 0.0;
+  double enuc_ = 0.0;
+
+  DeviceArray<double> boys_;
+  DeviceArray<double> S_, H_;
+  DeviceArray<double> B64_;
+  DeviceArray<float> B32_;
+  // Fock-build workspaces.
+  DeviceArray<double> dpacked_, rho_, rho_partial_, J_, K_, syrk_partial_, D_;
+  DeviceArray<double> c64_;
+  DeviceArray<float> c32_;
+  DeviceArray<uint16_t> ct_hi_, ct_lo_;
+  DeviceArray<char> U_;
+  int factor_cap_ = 0, ldc_ = 0, ni_ = 1;
+  OccupiedOperands ops_;
+  DeviceArray<int> flag_;
+};
+
+}  // namespace vqc

@@ -1,0 +1,38 @@
+// Small device kernels of the RHF SCF loop (all FP64).
+#pragma once
+
+#include "common.hpp"
+
+namespace vqc {
+
+// F = H + cj J + ck K.
+void assemble_fock(const double* H, const double* J, const double* K, double cj, double ck, std::size_t count,
+                   double* F, hipStream_t s);
+// out[0] = sum D .* A, out[1] = sum D .* B (both partial-sum reduced on device).
+void dot_pair(const double* D, const double* A, const double* B, std::size_t count, double* out, hipStream_t s);
+// out = sqrt(mean((a - b)^2)).
+void rms_difference(const double* a, const double* b, std::size_t count, double* out, hipStream_t s);
+// R = T - T^T (n x n).
+void antisymmetrise(const double* T, int n, double* R, hipStream_t s);
+// gram[row * cap + j] = gram[j * cap + row] = <E_row, E_j> for j < m (history slots).
+void diis_gram_row(const double* errors, std::size_t count, int row, int m, const int* slot_of, int cap,
+                   double* gram, hipStream_t s);
+// Solve the augmented Pulay system for m entries (order given by slot_of);
+// coef[j] for j < m; a singular system returns coef = e_{m-1}.
+void diis_solve(const double* gram, int m, const int* slot_of, int cap, double* coef, hipStream_t s);
+// F = sum_j coef[j] * focks[slot_of[j]].
+void diis_extrapolate(const double* focks, std::size_t count, int m, const int* slot_of, const double* coef,
+                      double* F, hipStream_t s);
+// Column scale: W[:, j] = V[:, j] * f(w[j]) with f = w^-1/2 (column-major n x n).
+void scale_columns_inv_sqrt(const double* V, const double* w, int n, double* W, hipStream_t s);
+// Cocc (row-major n x ldc) = scale * C[:, :nocc] (C column-major n x n), zero padded.
+void occupied_from_orbitals(const double* C, int n, int nocc, int ldc, double scale, double* Cocc, hipStream_t s);
+// Cocc (row-major n x ldc) from selected eigenvectors: column i <- sqrt|w[idx[i]]| V[:, idx[i]].
+void factor_from_eigen(const double* V, const double* w, const int* idx, int n, int r, int ldc, double* Cocc,
+                       hipStream_t s);
+// Non-finite detector: flag[0] |= any non-finite in x.
+void check_finite(const double* x, std::size_t count, int* flag, hipStream_t s);
+// Symmetric copy from a column-major Cholesky factor: diag[i] = L[i + i*n].
+void extract_diagonal(const double* A, int n, double* diag, hipStream_t s);
+
+}  // namespace vqc
