@@ -6,7 +6,9 @@
 //   BF16X3 v_mfma_f32_16x16x32_bf16 (8 bf16 per lane, K = 32); each FP32
 //          operand x is split as hi = bf16(x), lo = bf16(x - hi) and the
 //          product is accumulated as hi*hi + hi*lo + lo*hi in FP32.
-// The 16x16 accumulator holds D[4*(lane/16) + r][lane % 16] in register r.
+// Accumulator layout (measured on gfx950): column lane % 16 in every case;
+// row 4*(lane/16) + r in register r for the FP32 and BF16 instructions, but
+// row lane/16 + 4*r for v_mfma_f64_16x16x4_f64 (AccRow below).
 #include "jk.hpp"
 
 #include <algorithm>
@@ -29,6 +31,16 @@ template <>
 struct AccOf<float> {
   using type = f4;
 };
+
+// Row of accumulator register r held by `lane` in a 16x16 MFMA tile.
+template <class T>
+__device__ __forceinline__ int acc_row(int lane, int r) {
+  return 4 * (lane >> 4) + r;
+}
+template <>
+__device__ __forceinline__ int acc_row<double>(int lane, int r) {
+  return (lane >> 4) + 4 * r;
+}
 
 __device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -193,10 +205,9 @@ __global__ __launch_bounds__(256) void occupied_u_kernel(const T* __restrict__ B
     for (int b = 0; b < NI; ++b) {
       const int i = i0 + b * 16 + (lane & 15);
       if (i >= r) continue;
-      const int q = q0 + wave * 32 + a * 16 + 4 * (lane >> 4);
-      T* dst = U + (static_cast<std::size_t>(mu) * r + i) * ldq + q;
+      T* dst = U + (static_cast<std::size_t>(mu) * r + i) * ldq + q0 + wave * 32 + a * 16;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) dst[v] = acc[a][b][v];
+      for (int v = 0; v < 4; ++v) dst[acc_row<T>(lane, v)] = acc[a][b][v];
     }
 }
 
@@ -349,7 +360,7 @@ __global__ __launch_bounds__(256) void exchange_syrk_kernel(const T* __restrict_
       const int lam = l0 + wn * 32 + y * 16 + (lane & 15);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int mu = m0 + wm * 32 + x * 16 + 4 * (lane >> 4) + v;
+        const int mu = m0 + wm * 32 + x * 16 + acc_row<T>(lane, v);
         if (mu < n && lam < n) out[static_cast<std::size_t>(mu) * n + lam] = sign * static_cast<double>(acc[x][y][v]);
       }
     }
