@@ -173,11 +173,13 @@ GENERATIVEQC_API generativeqc_status vibeqc_fock_plan_evaluate_occupied(
     generativeqc_fock_plan* plan, const double* occupied, uint64_t nbf, uint64_t nocc,
     double* coulomb, double* exchange);
 
-/* Device-resident timing harness: uploads C once (as above), then runs
- * `repeat` complete J+K builds on the device with no host transfer in
- * between and synchronises.  kernel_ms (may be NULL) receives, per build
- * phase, the average device time measured with HIP events on the plan's
- * stream: [0] U = B C (+rho), [1] K = U U^T, [2] J = B rho, [3] whole build. */
+/* Device-resident timing harness: uploads C once (as above), runs one
+ * untimed build, synchronises, then runs `repeat` complete J+K builds on the
+ * device with no host transfer in between and synchronises again.
+ * kernel_ms (may be NULL, else 5 entries) receives per-build averages:
+ * device times from HIP events on the plan's stream, [0] U = B C,
+ * [1] K = U U^T, [2] J = B rho, [3] whole build, and [4] the host wall time
+ * between the two synchronisations divided by `repeat`. */
 GENERATIVEQC_API generativeqc_status vibeqc_fock_plan_benchmark_occupied(
     generativeqc_fock_plan* plan, const double* occupied, uint64_t nbf, uint64_t nocc,
     uint32_t repeat, double* kernel_ms);

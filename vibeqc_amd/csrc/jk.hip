@@ -217,8 +217,8 @@ template <int NI>
 __global__ __launch_bounds__(256) void occupied_u_bf16x3_kernel(const float* __restrict__ B,
                                                                const __bf16* __restrict__ Cth,
                                                                const __bf16* __restrict__ Ctl,
-                                                               float* __restrict__ U, int n, int ldq, int ldct,
-                                                               int r, int nqb) {
+                                                               __bf16* __restrict__ Uh, __bf16* __restrict__ Ul,
+                                                               int n, int ldq, int ldct, int r, int nqb) {
   constexpr int BM = 128, BK = 32, BN = 16 * NI, PAD = 8;
   __shared__ __bf16 Xh[BM][BK + PAD], Xl[BM][BK + PAD];
   __shared__ __bf16 Yh[BN][BK + PAD], Yl[BN][BK + PAD];
@@ -283,10 +283,9 @@ __global__ __launch_bounds__(256) void occupied_u_bf16x3_kernel(const float* __r
     for (int b = 0; b < NI; ++b) {
       const int i = i0 + b * 16 + (lane & 15);
       if (i >= r) continue;
-      const int q = q0 + wave * 32 + a * 16 + 4 * (lane >> 4);
-      float* dst = U + (static_cast<std::size_t>(mu) * r + i) * ldq + q;
+      const std::size_t off = (static_cast<std::size_t>(mu) * r + i) * ldq + q0 + wave * 32 + a * 16 + 4 * (lane >> 4);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) dst[v] = acc[a][b][v];
+      for (int v = 0; v < 4; ++v) split_bf16(acc[a][b][v], Uh[off + v], Ul[off + v]);
     }
 }
 
@@ -295,139 +294,189 @@ __global__ __launch_bounds__(256) void occupied_u_bf16x3_kernel(const float* __r
 // Block: 256 threads = 2x2 waves, tile 64 x 64, partial results per split.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ void tri_tile(int t, int& bi, int& bj) {
-  int i = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-  while (i * (i + 1) / 2 > t) --i;
-  while ((i + 1) * (i + 2) / 2 <= t) ++i;
-  bi = i;
-  bj = t - i * (i + 1) / 2;
+
+// FP64 / FP32 SYRK with the same block shape as the BF16X3 one (8 waves, tile
+// 256 x 128, 64 x 64 per wave as 4 x 4 MFMA 16x16x4 tiles).  LDS rows keep W's
+// K-contiguous order with one element of padding, so the 16 rows a wave reads
+// at one k fall on distinct banks.
+template <class T>
+__global__ __launch_bounds__(512) void exchange_syrk_kernel(const T* __restrict__ W, std::size_t kd, int n,
+                                                           int ntiles, int ncb, int nwork, std::size_t kbegin,
+                                                           std::size_t kend, std::size_t kchunk, double sign,
+                                                           double* __restrict__ partial);
+
+// BF16X3 SYRK over the pre-split hi/lo planes written by occupied_u.
+// Block: 8 waves (4 x 2), tile 256 (mu) x 128 (lambda); each wave owns 64 x 64
+// as 4 x 4 MFMA 16x16x32 tiles, so one LDS k-step of 32 feeds 48 MFMAs per
+// wave.  Row block rb covers min(2 (rb + 1), ncb) column blocks (the lower
+// triangle).  Block ids are remapped so the blocks resident on one XCD
+// (hardware dispatch is round-robin over the 8 XCDs) work on the same k-range
+// of W and share that XCD's L2.
+constexpr int kSyrkTM = 256, kSyrkTN = 128;
+
+__device__ __forceinline__ void syrk_tile_of(int t, int ncb, int& rb, int& cb) {
+  int base = 0;
+  rb = 0;
+  for (;;) {
+    const int cnt = min(2 * (rb + 1), ncb);
+    if (t < base + cnt) {
+      cb = t - base;
+      return;
+    }
+    base += cnt;
+    ++rb;
+  }
+}
+
+__global__ __launch_bounds__(512) void exchange_syrk_bf16x3_kernel(const __bf16* __restrict__ Wh,
+                                                                  const __bf16* __restrict__ Wl, std::size_t kd,
+                                                                  int n, int ntiles, int ncb, int nwork,
+                                                                  std::size_t kbegin, std::size_t kend,
+                                                                  std::size_t kchunk, double sign,
+                                                                  double* __restrict__ partial) {
+  constexpr int TM = kSyrkTM, TN = kSyrkTN, BK = 32, PAD = 8, NT = 512;
+  __shared__ __bf16 Ah[TM][BK + PAD], Al[TM][BK + PAD];
+  __shared__ __bf16 Bh[TN][BK + PAD], Bl[TN][BK + PAD];
+  const int per = gridDim.x / 8;
+  const int work = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (work >= nwork) return;
+  const int split = work / ntiles, tile = work % ntiles;
+  int rb, cb;
+  syrk_tile_of(tile, ncb, rb, cb);
+  const int m0 = rb * TM, l0 = cb * TN;
+  const std::size_t k0 = kbegin + static_cast<std::size_t>(split) * kchunk;
+  const std::size_t k1 = k0 + kchunk < kend ? k0 + kchunk : kend;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  f4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4{};
+  const bf8 zero = bf8{};
+  for (std::size_t kb = k0; kb < k1; kb += BK) {
+    for (int c = tid; c < TM * 4; c += NT) {
+      const int row = c >> 2, kc = (c & 3) << 3, grow = m0 + row;
+      bf8 h = zero, l = zero;
+      if (grow < n) {
+        const std::size_t off = static_cast<std::size_t>(grow) * kd + kb + kc;
+        h = *reinterpret_cast<const bf8*>(Wh + off);
+        l = *reinterpret_cast<const bf8*>(Wl + off);
+      }
+      *reinterpret_cast<bf8*>(&Ah[row][kc]) = h;
+      *reinterpret_cast<bf8*>(&Al[row][kc]) = l;
+    }
+    for (int c = tid; c < TN * 4; c += NT) {
+      const int row = c >> 2, kc = (c & 3) << 3, grow = l0 + row;
+      bf8 h = zero, l = zero;
+      if (grow < n) {
+        const std::size_t off = static_cast<std::size_t>(grow) * kd + kb + kc;
+        h = *reinterpret_cast<const bf8*>(Wh + off);
+        l = *reinterpret_cast<const bf8*>(Wl + off);
+      }
+      *reinterpret_cast<bf8*>(&Bh[row][kc]) = h;
+      *reinterpret_cast<bf8*>(&Bl[row][kc]) = l;
+    }
+    __syncthreads();
+    const int kq = 8 * (lane >> 4);
+    bf8 bh[4], bl[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      bh[j] = *reinterpret_cast<const bf8*>(&Bh[wc * 64 + j * 16 + (lane & 15)][kq]);
+      bl[j] = *reinterpret_cast<const bf8*>(&Bl[wc * 64 + j * 16 + (lane & 15)][kq]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const bf8 ah = *reinterpret_cast<const bf8*>(&Ah[wr * 64 + i * 16 + (lane & 15)][kq]);
+      const bf8 al = *reinterpret_cast<const bf8*>(&Al[wr * 64 + i * 16 + (lane & 15)][kq]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        acc[i][j] = mfma(ah, bh[j], acc[i][j]);
+        acc[i][j] = mfma(ah, bl[j], acc[i][j]);
+        acc[i][j] = mfma(al, bh[j], acc[i][j]);
+      }
+    }
+    __syncthreads();
+  }
+  double* out = partial + static_cast<std::size_t>(split) * n * n;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int lam = l0 + wc * 64 + j * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int mu = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + v;
+        if (mu < n && lam < n && lam <= mu)
+          out[static_cast<std::size_t>(mu) * n + lam] = sign * static_cast<double>(acc[i][j][v]);
+      }
+    }
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void exchange_syrk_kernel(const T* __restrict__ W, std::size_t kd, int n,
-                                                           std::size_t kbegin, std::size_t kend,
-                                                           std::size_t kchunk, double sign,
+__global__ __launch_bounds__(512) void exchange_syrk_kernel(const T* __restrict__ W, std::size_t kd, int n,
+                                                           int ntiles, int ncb, int nwork, std::size_t kbegin,
+                                                           std::size_t kend, std::size_t kchunk, double sign,
                                                            double* __restrict__ partial) {
   using Acc = typename AccOf<T>::type;
-  constexpr int BK = 16, BT = 64, PAD = 1;
-  __shared__ T As[BK][BT + PAD];
-  __shared__ T Bs[BK][BT + PAD];
-  int bi, bj;
-  tri_tile(blockIdx.x, bi, bj);
-  const int m0 = bi * BT, l0 = bj * BT;
-  const std::size_t k0 = kbegin + static_cast<std::size_t>(blockIdx.y) * kchunk;
+  constexpr int TM = kSyrkTM, TN = kSyrkTN, BK = 16, PAD = 1, NT = 512, V = 16 / sizeof(T);
+  __shared__ T As[TM][BK + PAD];
+  __shared__ T Bs[TN][BK + PAD];
+  const int per = gridDim.x / 8;
+  const int work = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (work >= nwork) return;
+  const int split = work / ntiles, tile = work % ntiles;
+  int rb, cb;
+  syrk_tile_of(tile, ncb, rb, cb);
+  const int m0 = rb * TM, l0 = cb * TN;
+  const std::size_t k0 = kbegin + static_cast<std::size_t>(split) * kchunk;
   const std::size_t k1 = k0 + kchunk < kend ? k0 + kchunk : kend;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
-  Acc acc[2][2];
+  const int wr = wave >> 1, wc = wave & 1;
+  Acc acc[4][4];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = Acc{};
-  const int lr = tid >> 2, lk = (tid & 3) * 4;
+    for (int j = 0; j < 4; ++j) acc[i][j] = Acc{};
+  constexpr int CH = BK / V;  // 16-byte chunks per row
   for (std::size_t kb = k0; kb < k1; kb += BK) {
-    {
-      const int mrow = m0 + lr, lrow = l0 + lr;
+    for (int c = tid; c < TM * CH; c += NT) {
+      const int row = c / CH, kc = (c % CH) * V, grow = m0 + row;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const std::size_t k = kb + lk + j;
-        As[lk + j][lr] = (mrow < n && k < k1) ? W[static_cast<std::size_t>(mrow) * kd + k] : T(0);
-        Bs[lk + j][lr] = (lrow < n && k < k1) ? W[static_cast<std::size_t>(lrow) * kd + k] : T(0);
-      }
+      for (int e = 0; e < V; ++e) As[row][kc + e] = grow < n ? W[static_cast<std::size_t>(grow) * kd + kb + kc + e] : T(0);
+    }
+    for (int c = tid; c < TN * CH; c += NT) {
+      const int row = c / CH, kc = (c % CH) * V, grow = l0 + row;
+#pragma unroll
+      for (int e = 0; e < V; ++e) Bs[row][kc + e] = grow < n ? W[static_cast<std::size_t>(grow) * kd + kb + kc + e] : T(0);
     }
     __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int k = kk + (lane >> 4);
-      T a[2], b[2];
+      T b[4];
 #pragma unroll
-      for (int x = 0; x < 2; ++x) {
-        a[x] = As[k][wm * 32 + x * 16 + (lane & 15)];
-        b[x] = Bs[k][wn * 32 + x * 16 + (lane & 15)];
+      for (int j = 0; j < 4; ++j) b[j] = Bs[wc * 64 + j * 16 + (lane & 15)][k];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const T a = As[wr * 64 + i * 16 + (lane & 15)][k];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a, b[j], acc[i][j]);
       }
-#pragma unroll
-      for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y) acc[x][y] = mfma(a[x], b[y], acc[x][y]);
     }
     __syncthreads();
   }
-  double* out = partial + static_cast<std::size_t>(blockIdx.y) * n * n;
+  double* out = partial + static_cast<std::size_t>(split) * n * n;
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int y = 0; y < 2; ++y) {
-      const int lam = l0 + wn * 32 + y * 16 + (lane & 15);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int mu = m0 + wm * 32 + x * 16 + acc_row<T>(lane, v);
-        if (mu < n && lam < n) out[static_cast<std::size_t>(mu) * n + lam] = sign * static_cast<double>(acc[x][y][v]);
-      }
-    }
-}
-
-__global__ __launch_bounds__(256) void exchange_syrk_bf16x3_kernel(const float* __restrict__ W, std::size_t kd,
-                                                                  int n, std::size_t kbegin, std::size_t kend,
-                                                                  std::size_t kchunk, double sign,
-                                                                  double* __restrict__ partial) {
-  constexpr int BK = 32, BT = 64, PAD = 8;
-  __shared__ __bf16 Ah[BT][BK + PAD], Al[BT][BK + PAD];
-  __shared__ __bf16 Bh[BT][BK + PAD], Bl[BT][BK + PAD];
-  int bi, bj;
-  tri_tile(blockIdx.x, bi, bj);
-  const int m0 = bi * BT, l0 = bj * BT;
-  const std::size_t k0 = kbegin + static_cast<std::size_t>(blockIdx.y) * kchunk;
-  const std::size_t k1 = k0 + kchunk < kend ? k0 + kchunk : kend;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
-  f4 acc[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = f4{};
-  const int lr = tid >> 2, lk = (tid & 3) * 8;
-  for (std::size_t kb = k0; kb < k1; kb += BK) {
-    {
-      const int mrow = m0 + lr, lrow = l0 + lr;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const std::size_t k = kb + lk + j;
-        const float a = (mrow < n && k < k1) ? W[static_cast<std::size_t>(mrow) * kd + k] : 0.0f;
-        const float b = (lrow < n && k < k1) ? W[static_cast<std::size_t>(lrow) * kd + k] : 0.0f;
-        split_bf16(a, Ah[lr][lk + j], Al[lr][lk + j]);
-        split_bf16(b, Bh[lr][lk + j], Bl[lr][lk + j]);
-      }
-    }
-    __syncthreads();
-    const int kq = 8 * (lane >> 4);
-    bf8 ah[2], al[2], bh[2], bl[2];
-#pragma unroll
-    for (int x = 0; x < 2; ++x) {
-      ah[x] = *reinterpret_cast<const bf8*>(&Ah[wm * 32 + x * 16 + (lane & 15)][kq]);
-      al[x] = *reinterpret_cast<const bf8*>(&Al[wm * 32 + x * 16 + (lane & 15)][kq]);
-      bh[x] = *reinterpret_cast<const bf8*>(&Bh[wn * 32 + x * 16 + (lane & 15)][kq]);
-      bl[x] = *reinterpret_cast<const bf8*>(&Bl[wn * 32 + x * 16 + (lane & 15)][kq]);
-    }
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-      for (int y = 0; y < 2; ++y) {
-        acc[x][y] = mfma(ah[x], bh[y], acc[x][y]);
-        acc[x][y] = mfma(ah[x], bl[y], acc[x][y]);
-        acc[x][y] = mfma(al[x], bh[y], acc[x][y]);
-      }
-    __syncthreads();
-  }
-  double* out = partial + static_cast<std::size_t>(blockIdx.y) * n * n;
-#pragma unroll
-  for (int x = 0; x < 2; ++x)
-#pragma unroll
-    for (int y = 0; y < 2; ++y) {
-      const int lam = l0 + wn * 32 + y * 16 + (lane & 15);
+    for (int j = 0; j < 4; ++j) {
+      const int lam = l0 + wc * 64 + j * 16 + (lane & 15);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int mu = m0 + wm * 32 + x * 16 + 4 * (lane >> 4) + v;
-        if (mu < n && lam < n) out[static_cast<std::size_t>(mu) * n + lam] = sign * static_cast<double>(acc[x][y][v]);
+        const int mu = m0 + wr * 64 + i * 16 + acc_row<T>(lane, v);
+        if (mu < n && lam < n && lam <= mu)
+          out[static_cast<std::size_t>(mu) * n + lam] = sign * static_cast<double>(acc[i][j][v]);
       }
     }
 }
@@ -476,13 +525,12 @@ void launch_u(Precision prec, const void* B, const OccupiedOperands& ops, int ld
   } else {
     hipLaunchKernelGGL((occupied_u_bf16x3_kernel<NI>), grid, dim3(256), 0, s, static_cast<const float*>(B),
                        static_cast<const __bf16*>(ops.ct_hi), static_cast<const __bf16*>(ops.ct_lo),
-                       static_cast<float*>(U), ops.n, ldq, ops.ldct, ops.r, nqb);
+                       static_cast<__bf16*>(U), static_cast<__bf16*>(U) + static_cast<std::size_t>(ops.n) * ops.r * ldq,
+                       ops.n, ldq, ops.ldct, ops.r, nqb);
   }
   VQC_LAUNCH("occupied_u_kernel");
 }
 
-constexpr int kSyrkTargetBlocks = 2048;
-constexpr int kMaxSplits = 64;
 
 }  // namespace
 
@@ -579,34 +627,54 @@ void occupied_u(Precision prec, const void* B, const OccupiedOperands& ops, int 
   }
 }
 
+namespace {
+// Split count of the large-tile BF16X3 SYRK: about 4 blocks per CU, with the
+// partial slices of one run capped at 256 MiB.
+int syrk_split_cap(int n) {
+  const std::size_t slice = static_cast<std::size_t>(n) * n * sizeof(double);
+  return static_cast<int>(std::max<std::size_t>(1, std::min<std::size_t>(256, (256ull << 20) / slice)));
+}
+int syrk_tiles(int n, int& ncb) {
+  ncb = (n + kSyrkTN - 1) / kSyrkTN;
+  const int nrb = (n + kSyrkTM - 1) / kSyrkTM;
+  int t = 0;
+  for (int rb = 0; rb < nrb; ++rb) t += std::min(2 * (rb + 1), ncb);
+  return t;
+}
+}  // namespace
+
 std::size_t exchange_partial_size(int n) {
-  return static_cast<std::size_t>(kMaxSplits) * 2 * static_cast<std::size_t>(n) * n;
+  return static_cast<std::size_t>(syrk_split_cap(n) + 1) * 2 * static_cast<std::size_t>(n) * n;
 }
 
 void exchange_syrk(Precision prec, const void* U, int n, int r, int r_pos, int ldq, double* K, double* partial,
                    hipStream_t s) {
-  const int nb = (n + 63) / 64;
-  const int tiles = nb * (nb + 1) / 2;
   const std::size_t kd = static_cast<std::size_t>(r) * ldq;
-  const std::size_t bk = prec == Precision::BF16X3 ? 32 : 16;
   int slices = 0;
   auto run = [&](std::size_t kb, std::size_t ke, double sign) {
     if (ke <= kb) return;
     const std::size_t len = ke - kb;
-    int splits = std::max(1, std::min(kMaxSplits, kSyrkTargetBlocks / tiles));
-    std::size_t chunk = round_up((len + splits - 1) / splits, bk);
-    splits = static_cast<int>((len + chunk - 1) / chunk);
-    dim3 grid(tiles, splits);
     double* out = partial + static_cast<std::size_t>(slices) * n * n;
-    if (prec == Precision::FP64)
-      hipLaunchKernelGGL(exchange_syrk_kernel<double>, grid, dim3(256), 0, s, static_cast<const double*>(U), kd, n,
+    int ncb = 0;
+    const int tiles = syrk_tiles(n, ncb);
+    const std::size_t bk = prec == Precision::BF16X3 ? 32 : 16;
+    int splits = std::max(1, std::min(syrk_split_cap(n), (1024 + tiles - 1) / tiles));
+    const std::size_t chunk = round_up((len + splits - 1) / splits, bk);
+    splits = static_cast<int>((len + chunk - 1) / chunk);
+    const int nwork = tiles * splits;
+    const unsigned grid = static_cast<unsigned>(round_up(nwork, 8));
+    if (prec == Precision::BF16X3) {
+      const __bf16* wh = static_cast<const __bf16*>(U);
+      const __bf16* wl = wh + static_cast<std::size_t>(n) * kd;
+      hipLaunchKernelGGL(exchange_syrk_bf16x3_kernel, dim3(grid), dim3(512), 0, s, wh, wl, kd, n, tiles, ncb, nwork,
                          kb, ke, chunk, sign, out);
-    else if (prec == Precision::FP32)
-      hipLaunchKernelGGL(exchange_syrk_kernel<float>, grid, dim3(256), 0, s, static_cast<const float*>(U), kd, n,
-                         kb, ke, chunk, sign, out);
-    else
-      hipLaunchKernelGGL(exchange_syrk_bf16x3_kernel, grid, dim3(256), 0, s, static_cast<const float*>(U), kd, n,
-                         kb, ke, chunk, sign, out);
+    } else if (prec == Precision::FP64) {
+      hipLaunchKernelGGL(exchange_syrk_kernel<double>, dim3(grid), dim3(512), 0, s, static_cast<const double*>(U), kd,
+                         n, tiles, ncb, nwork, kb, ke, chunk, sign, out);
+    } else {
+      hipLaunchKernelGGL(exchange_syrk_kernel<float>, dim3(grid), dim3(512), 0, s, static_cast<const float*>(U), kd,
+                         n, tiles, ncb, nwork, kb, ke, chunk, sign, out);
+    }
     VQC_LAUNCH("exchange_syrk_kernel");
     slices += splits;
   };

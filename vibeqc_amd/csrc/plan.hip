@@ -16,7 +16,10 @@
 #include "plan.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <limits>
 #include <numeric>
 #include <stdexcept>
@@ -37,6 +40,33 @@ struct Event {
   Event(const Event&) = delete;
   Event& operator=(const Event&) = delete;
   Event(Event&& o) noexcept : e(std::exchange(o.e, nullptr)) {}
+};
+
+// Opt-in phase trace: with VIBEQC_TRACE set, one JSON line per phase on stderr
+// (the role of the reference's GENERATIVEQC_DF_TRACE, cuda_component_trace.cpp).
+class Trace {
+ public:
+  explicit Trace(const char* scope) : scope_(scope), on_(std::getenv("VIBEQC_TRACE") != nullptr) {
+    last_ = start_ = std::chrono::steady_clock::now();
+  }
+  void mark(const char* phase) {
+    if (!on_) return;
+    const auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "{\"trace\": \"%s\", \"phase\": \"%s\", \"ms\": %.3f}\n", scope_, phase,
+                 std::chrono::duration<double, std::milli>(now - last_).count());
+    last_ = now;
+  }
+  void finish(int n, int naux, std::size_t npair) {
+    if (!on_) return;
+    std::fprintf(stderr, "{\"trace\": \"%s\", \"phase\": \"total\", \"ms\": %.3f, \"nbf\": %d, \"naux\": %d, \"npair\": %zu}\n",
+                 scope_, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start_).count(), n,
+                 naux, npair);
+  }
+
+ private:
+  const char* scope_;
+  bool on_;
+  std::chrono::steady_clock::time_point start_, last_;
 };
 
 void check_basis_limits(const System& sys, int lmax, const char* what) {
@@ -90,11 +120,13 @@ void DfPlan::setup(const System& aux) {
   nocc_ = orb_.electrons / 2;
   flag_.resize(1);
 
+  Trace trace("plan_setup");
   boys_.upload(boys_table_host(), stream_);
   DeviceBasis ob;
   upload_basis(orb_, ob, false, stream_);
   AuxiliaryLayout al;
   build_auxiliary_layout(aux, al, stream_);
+  trace.mark("basis_upload");
 
   const std::size_t nn = static_cast<std::size_t>(n_) * n_;
   S_.resize(nn);
@@ -105,16 +137,24 @@ void DfPlan::setup(const System& aux) {
     assemble_fock(T.data(), V.data(), V.data(), 1.0, 0.0, nn, H_.data(), stream_);
     sync();
   }
+  trace.mark("one_electron");
   DeviceArray<double> M(static_cast<std::size_t>(naux_) * naux_);
   compute_metric(al, boys_.data(), M.data(), stream_);
+  sync();
+  trace.mark("metric");
   B64_.resize(npair_ * ldq_);
   compute_three_center(orb_, ob, al, boys_.data(), B64_.data(), ldq_, opts_.screening * 1e-10, stream_);
+  sync();
+  trace.mark("three_center");
   factor_metric(M, B64_);
+  trace.mark(cholesky_ ? "cholesky_trsm" : "eigen_cutoff_gemm");
   VQC_HIP(hipMemsetAsync(flag_.data(), 0, sizeof(int), stream_));
   check_finite(B64_.data(), B64_.size(), flag_.data(), stream_);
   int bad = 0;
   flag_.download(&bad, 1, stream_);
   sync();
+  trace.mark("finite_check");
+  trace.finish(n_, naux_, npair_);
   if (bad) throw NumericalFailure("non-finite fitted three-centre tensor");
 }
 
@@ -366,7 +406,7 @@ JkResult DfPlan::evaluate_occupied(const std::vector<double>& occupied, int nocc
   return out;
 }
 
-std::array<double, 4> DfPlan::benchmark_occupied(const std::vector<double>& occupied, int nocc, unsigned repeat) {
+std::array<double, 5> DfPlan::benchmark_occupied(const std::vector<double>& occupied, int nocc, unsigned repeat) {
   VQC_HIP(hipSetDevice(device_));
   if (repeat == 0) throw std::invalid_argument("repeat must be positive");
   if (nocc <= 0 || occupied.size() != static_cast<std::size_t>(n_) * nocc)
@@ -379,6 +419,7 @@ std::array<double, 4> DfPlan::benchmark_occupied(const std::vector<double>& occu
   sync();
   const void* B = p == Precision::FP64 ? static_cast<const void*>(B64_.data()) : static_cast<const void*>(B32_.data());
   std::vector<Event> ev(static_cast<std::size_t>(repeat) * 4);
+  const auto t0 = std::chrono::steady_clock::now();
   for (unsigned it = 0; it < repeat; ++it) {
     VQC_HIP(hipEventRecord(ev[it * 4 + 0].e, stream_));
     ops_.r = nocc;
@@ -393,7 +434,8 @@ std::array<double, 4> DfPlan::benchmark_occupied(const std::vector<double>& occu
     VQC_HIP(hipEventRecord(ev[it * 4 + 3].e, stream_));
   }
   sync();
-  std::array<double, 4> ms{0.0, 0.0, 0.0, 0.0};
+  const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  std::array<double, 5> ms{0.0, 0.0, 0.0, 0.0, 0.0};
   for (unsigned it = 0; it < repeat; ++it) {
     float a = 0, b = 0, c = 0, d = 0;
     VQC_HIP(hipEventElapsedTime(&a, ev[it * 4 + 0].e, ev[it * 4 + 1].e));
@@ -405,6 +447,7 @@ std::array<double, 4> DfPlan::benchmark_occupied(const std::vector<double>& occu
     ms[2] += c;
     ms[3] += d;
   }
+  ms[4] = wall;
   for (auto& x : ms) x /= repeat;
   return ms;
 }

@@ -70,7 +70,7 @@ class DfPlan {
   // Raw J/K for D = 2 C C^T (C row-major n x nocc, host).
   JkResult evaluate_occupied(const std::vector<double>& occupied, int nocc);
   // Timing harness (see vibeqc_fock_plan_benchmark_occupied).
-  std::array<double, 4> benchmark_occupied(const std::vector<double>& occupied, int nocc, unsigned repeat);
+  std::array<double, 5> benchmark_occupied(const std::vector<double>& occupied, int nocc, unsigned repeat);
 
   ScfOutcome solve(const ScfOptions& options, const std::vector<double>* initial_density);
 

@@ -338,7 +338,7 @@ class FockPlan:
         """Device-timed Fock builds from a resident occupied factor (ms per phase)."""
         n = self.nao
         c = np.array(occupied, dtype=np.float64, order="C", copy=True)
-        ms = (ct.c_double * 4)()
+        ms = (ct.c_double * 5)()
         with self._lock:
             self._ensure_open()
             self._check(
@@ -346,7 +346,7 @@ class FockPlan:
                     self._handle, _native.pointer(c), n, c.shape[1], int(repeat), ms
                 )
             )
-        return {"exchange_u_ms": ms[0], "exchange_syrk_ms": ms[1], "coulomb_ms": ms[2], "build_ms": ms[3]}
+        return {"exchange_u_ms": ms[0], "exchange_syrk_ms": ms[1], "coulomb_ms": ms[2], "build_ms": ms[3], "wall_ms": ms[4]}
 
     def solve(
         self,
