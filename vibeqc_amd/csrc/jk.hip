@@ -146,146 +146,198 @@ __global__ __launch_bounds__(256) void coulomb_matrix_kernel(const T* __restrict
 
 // ---------------------------------------------------------------------------
 // K, step 1: U[mu][i][Q] = sum_nu B[(mu,nu)][Q] C[nu][i]
-// Block: 256 threads = 4 waves stacked along Q; tile 128 (Q) x 16*NI (i).
+// Block: 2 x WQ waves (WQ along Q x 2 along i); tile 32*WQ (Q) x 32*NIW (i),
+// i.e. all occupied columns up to 320 in one block, so each packed row of B
+// is streamed once per mu.  Every wave owns 2 x NIW MFMA 16x16 tiles.  FP32 and
+// BF16X3 use WQ = 8 (1024 threads, 256 rows of Q per C tile); FP64 uses
+// WQ = 4 so its 2 x NIW x 4 double accumulators fit the register budget.
 // ---------------------------------------------------------------------------
 
-template <class T, int NI>
-__global__ __launch_bounds__(256) void occupied_u_kernel(const T* __restrict__ B, const T* __restrict__ C,
-                                                        T* __restrict__ U, int n, int ldq, int ldc, int r,
-                                                        int nqb) {
+constexpr int kUTileQ = 256;
+
+template <class T, int NIW, int WQ>
+__global__ __launch_bounds__(128 * WQ) void occupied_u_kernel(const T* __restrict__ B, const T* __restrict__ C,
+                                                             T* __restrict__ U, int n, int ldq, int ldc, int r,
+                                                             int nqb) {
   using Acc = typename AccOf<T>::type;
-  constexpr int BM = 128, BK = 16, BN = 16 * NI;
-  __shared__ T Xs[BK][BM];
-  __shared__ T Ys[BK][BN];
+  constexpr int BQ = 32 * WQ, BK = 16, BI = 32 * NIW, NT = 128 * WQ;
+  constexpr int CP = (BK * BI + NT - 1) / NT;  // C elements staged per thread
+  __shared__ T Bs[2][BK][BQ];
+  __shared__ T Cs[2][BK][BI];
   const int mu = blockIdx.x / nqb;
-  const int q0 = (blockIdx.x % nqb) * BM;
-  const int i0 = blockIdx.y * BN;
+  const int q0 = (blockIdx.x % nqb) * BQ;
+  const int i0 = blockIdx.y * BI;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  Acc acc[2][NI];
+  const int wq = wave % WQ, wi = wave / WQ;
+  Acc acc[2][NIW];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int b = 0; b < NI; ++b) acc[a][b] = Acc{};
-  const int xk = tid >> 4, xm = (tid & 15) * 8;
+    for (int y = 0; y < NIW; ++y) acc[x][y] = Acc{};
+  const int bk = tid / (BQ / 4), bq = (tid % (BQ / 4)) * 4;
+  T rb[4], rc[CP];
+  // Stage the next k-step in registers so its global latency overlaps the
+  // MFMAs of the current one; LDS is double-buffered (one barrier per step).
+  auto gload = [&](int nu0) {
+    const int nu = nu0 + bk, q = q0 + bq;
+    if (nu < n && q < ldq) {
+      const T* src = B + packed_index(mu, nu) * ldq + q;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) rb[e] = src[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) rb[e] = T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < CP; ++j) {
+      const int e = tid + j * NT;
+      const int k = e / BI, c = e % BI, nu2 = nu0 + k;
+      rc[j] = (e < BK * BI && nu2 < n) ? C[static_cast<std::size_t>(nu2) * ldc + i0 + c] : T(0);
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) Bs[buf][bk][bq + e] = rb[e];
+#pragma unroll
+    for (int j = 0; j < CP; ++j) {
+      const int e = tid + j * NT;
+      if (e < BK * BI) Cs[buf][e / BI][e % BI] = rc[j];
+    }
+  };
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  int cur = 0;
   for (int nu0 = 0; nu0 < n; nu0 += BK) {
-    {
-      const int nu = nu0 + xk;
-      if (nu < n) {
-        const T* src = B + packed_index(mu, nu) * ldq + q0 + xm;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) Xs[xk][xm + j] = src[j];
-      } else {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) Xs[xk][xm + j] = T(0);
-      }
-    }
-    for (int e = tid; e < BK * BN; e += 256) {
-      const int k = e / BN, c = e % BN;
-      const int nu = nu0 + k;
-      Ys[k][c] = nu < n ? C[static_cast<std::size_t>(nu) * ldc + i0 + c] : T(0);
-    }
-    __syncthreads();
+    const bool more = nu0 + BK < n;
+    if (more) gload(nu0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int k = kk + (lane >> 4);
-      T a0 = Xs[k][wave * 32 + (lane & 15)];
-      T a1 = Xs[k][wave * 32 + 16 + (lane & 15)];
+      const T a0 = Bs[cur][k][wq * 32 + (lane & 15)];
+      const T a1 = Bs[cur][k][wq * 32 + 16 + (lane & 15)];
 #pragma unroll
-      for (int b = 0; b < NI; ++b) {
-        const T bv = Ys[k][b * 16 + (lane & 15)];
-        acc[0][b] = mfma(a0, bv, acc[0][b]);
-        acc[1][b] = mfma(a1, bv, acc[1][b]);
+      for (int y = 0; y < NIW; ++y) {
+        const T bv = Cs[cur][k][wi * 16 * NIW + y * 16 + (lane & 15)];
+        acc[0][y] = mfma(a0, bv, acc[0][y]);
+        acc[1][y] = mfma(a1, bv, acc[1][y]);
       }
     }
+    if (more) sstore(cur ^ 1);
     __syncthreads();
+    cur ^= 1;
   }
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int b = 0; b < NI; ++b) {
-      const int i = i0 + b * 16 + (lane & 15);
-      if (i >= r) continue;
-      T* dst = U + (static_cast<std::size_t>(mu) * r + i) * ldq + q0 + wave * 32 + a * 16;
+    for (int y = 0; y < NIW; ++y) {
+      const int i = i0 + wi * 16 * NIW + y * 16 + (lane & 15);
+      const int qb = q0 + wq * 32 + x * 16;
+      if (i >= r || qb >= ldq) continue;
+      T* dst = U + (static_cast<std::size_t>(mu) * r + i) * ldq + qb;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) dst[acc_row<T>(lane, v)] = acc[a][b][v];
+      for (int v = 0; v < 4; ++v) dst[acc_row<T>(lane, v)] = acc[x][y][v];
     }
 }
 
-// BF16X3 variant: B and C are FP32; Ct_hi/Ct_lo are C^T split into bf16
-// halves (ldc rows of ldct K-contiguous entries, zero beyond n).
-template <int NI>
-__global__ __launch_bounds__(256) void occupied_u_bf16x3_kernel(const float* __restrict__ B,
-                                                               const __bf16* __restrict__ Cth,
-                                                               const __bf16* __restrict__ Ctl,
-                                                               __bf16* __restrict__ Uh, __bf16* __restrict__ Ul,
-                                                               int n, int ldq, int ldct, int r, int nqb) {
-  constexpr int BM = 128, BK = 32, BN = 16 * NI, PAD = 8;
-  __shared__ __bf16 Xh[BM][BK + PAD], Xl[BM][BK + PAD];
-  __shared__ __bf16 Yh[BN][BK + PAD], Yl[BN][BK + PAD];
+// BF16X3 variant: B (packed FP32) is split into hi/lo bfloat16 and transposed
+// into K-contiguous LDS rows: each thread gathers 8 consecutive nu of one Q
+// (a wave reads 64 consecutive Q per row, coalesced) and writes one 16-byte
+// chunk per plane.  C^T arrives pre-split (ldc rows of ldct K-contiguous
+// entries, zero beyond n).  U is written as hi/lo planes.  The next k-step is
+// staged in registers while the current one runs on the MFMAs.
+template <int NIW>
+__global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __restrict__ B,
+                                                                const __bf16* __restrict__ Cth,
+                                                                const __bf16* __restrict__ Ctl,
+                                                                __bf16* __restrict__ Uh, __bf16* __restrict__ Ul,
+                                                                int n, int ldq, int ldct, int r, int nqb) {
+  constexpr int BQ = kUTileQ, BK = 32, BI = 32 * NIW, PAD = 8, NT = 1024;
+  constexpr int YP = (BI * 4 + NT - 1) / NT;  // 16-byte C chunks per thread and plane
+  __shared__ __bf16 Xh[BQ][BK + PAD], Xl[BQ][BK + PAD];
+  __shared__ __bf16 Yh[BI][BK + PAD], Yl[BI][BK + PAD];
   const int mu = blockIdx.x / nqb;
-  const int q0 = (blockIdx.x % nqb) * BM;
-  const int i0 = blockIdx.y * BN;
+  const int q0 = (blockIdx.x % nqb) * BQ;
+  const int i0 = blockIdx.y * BI;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  f4 acc[2][NI];
+  const int wq = wave & 7, wi = wave >> 3;
+  f4 acc[2][NIW];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int b = 0; b < NI; ++b) acc[a][b] = f4{};
-  const int xk = tid >> 3, xm = (tid & 7) * 16;
-  for (int nu0 = 0; nu0 < n; nu0 += BK) {
-    {
-      const int nu = nu0 + xk;
-      if (nu < n) {
-        const float* src = B + packed_index(mu, nu) * ldq + q0 + xm;
+    for (int y = 0; y < NIW; ++y) acc[x][y] = f4{};
+  const int xq = tid % BQ, xg = tid / BQ;  // one Q column, 8 consecutive nu
+  float rx[8];
+  // Only the B gather (HBM) is staged in registers; the small C^T tile is
+  // L2-resident and is copied straight into LDS.
+  auto gload = [&](int nu0) {
+    const int q = q0 + xq;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-          __bf16 h, l;
-          split_bf16(src[j], h, l);
-          Xh[xm + j][xk] = h;
-          Xl[xm + j][xk] = l;
-        }
-      } else {
+    for (int e = 0; e < 8; ++e) {
+      const int nu = nu0 + xg * 8 + e;
+      rx[e] = (nu < n && q < ldq) ? B[packed_index(mu, nu) * ldq + q] : 0.0f;
+    }
+  };
+  auto sstore = [&](int nu0) {
+    bf8 h, l;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-          Xh[xm + j][xk] = static_cast<__bf16>(0.0f);
-          Xl[xm + j][xk] = static_cast<__bf16>(0.0f);
-        }
+    for (int e = 0; e < 8; ++e) {
+      __bf16 hh, ll;
+      split_bf16(rx[e], hh, ll);
+      h[e] = hh;
+      l[e] = ll;
+    }
+    *reinterpret_cast<bf8*>(&Xh[xq][xg * 8]) = h;
+    *reinterpret_cast<bf8*>(&Xl[xq][xg * 8]) = l;
+#pragma unroll
+    for (int j = 0; j < YP; ++j) {
+      const int e = tid + j * NT;
+      if (e < BI * 4) {
+        const std::size_t off = static_cast<std::size_t>(i0 + (e >> 2)) * ldct + nu0 + (e & 3) * 8;
+        *reinterpret_cast<bf8*>(&Yh[e >> 2][(e & 3) * 8]) = *reinterpret_cast<const bf8*>(Cth + off);
+        *reinterpret_cast<bf8*>(&Yl[e >> 2][(e & 3) * 8]) = *reinterpret_cast<const bf8*>(Ctl + off);
       }
     }
-    for (int e = tid; e < BN * 4; e += 256) {
-      const int row = e >> 2, kc = (e & 3) * 8;
-      const std::size_t off = static_cast<std::size_t>(i0 + row) * ldct + nu0 + kc;
-      *reinterpret_cast<bf8*>(&Yh[row][kc]) = *reinterpret_cast<const bf8*>(Cth + off);
-      *reinterpret_cast<bf8*>(&Yl[row][kc]) = *reinterpret_cast<const bf8*>(Ctl + off);
-    }
-    __syncthreads();
+  };
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int nu0 = 0; nu0 < n; nu0 += BK) {
+    const bool more = nu0 + BK < n;
+    if (more) gload(nu0 + BK);
     const int kq = 8 * (lane >> 4);
-    const bf8 ah0 = *reinterpret_cast<const bf8*>(&Xh[wave * 32 + (lane & 15)][kq]);
-    const bf8 al0 = *reinterpret_cast<const bf8*>(&Xl[wave * 32 + (lane & 15)][kq]);
-    const bf8 ah1 = *reinterpret_cast<const bf8*>(&Xh[wave * 32 + 16 + (lane & 15)][kq]);
-    const bf8 al1 = *reinterpret_cast<const bf8*>(&Xl[wave * 32 + 16 + (lane & 15)][kq]);
+    const bf8 ah0 = *reinterpret_cast<const bf8*>(&Xh[wq * 32 + (lane & 15)][kq]);
+    const bf8 al0 = *reinterpret_cast<const bf8*>(&Xl[wq * 32 + (lane & 15)][kq]);
+    const bf8 ah1 = *reinterpret_cast<const bf8*>(&Xh[wq * 32 + 16 + (lane & 15)][kq]);
+    const bf8 al1 = *reinterpret_cast<const bf8*>(&Xl[wq * 32 + 16 + (lane & 15)][kq]);
 #pragma unroll
-    for (int b = 0; b < NI; ++b) {
-      const bf8 bh = *reinterpret_cast<const bf8*>(&Yh[b * 16 + (lane & 15)][kq]);
-      const bf8 bl = *reinterpret_cast<const bf8*>(&Yl[b * 16 + (lane & 15)][kq]);
-      acc[0][b] = mfma(ah0, bh, acc[0][b]);
-      acc[0][b] = mfma(ah0, bl, acc[0][b]);
-      acc[0][b] = mfma(al0, bh, acc[0][b]);
-      acc[1][b] = mfma(ah1, bh, acc[1][b]);
-      acc[1][b] = mfma(ah1, bl, acc[1][b]);
-      acc[1][b] = mfma(al1, bh, acc[1][b]);
+    for (int y = 0; y < NIW; ++y) {
+      const int row = wi * 16 * NIW + y * 16 + (lane & 15);
+      const bf8 bh = *reinterpret_cast<const bf8*>(&Yh[row][kq]);
+      const bf8 bl = *reinterpret_cast<const bf8*>(&Yl[row][kq]);
+      acc[0][y] = mfma(ah0, bh, acc[0][y]);
+      acc[0][y] = mfma(ah0, bl, acc[0][y]);
+      acc[0][y] = mfma(al0, bh, acc[0][y]);
+      acc[1][y] = mfma(ah1, bh, acc[1][y]);
+      acc[1][y] = mfma(ah1, bl, acc[1][y]);
+      acc[1][y] = mfma(al1, bh, acc[1][y]);
     }
     __syncthreads();
+    if (more) {
+      sstore(nu0 + BK);
+      __syncthreads();
+    }
   }
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int b = 0; b < NI; ++b) {
-      const int i = i0 + b * 16 + (lane & 15);
-      if (i >= r) continue;
-      const std::size_t off = (static_cast<std::size_t>(mu) * r + i) * ldq + q0 + wave * 32 + a * 16 + 4 * (lane >> 4);
+    for (int y = 0; y < NIW; ++y) {
+      const int i = i0 + wi * 16 * NIW + y * 16 + (lane & 15);
+      const int qb = q0 + wq * 32 + x * 16;
+      if (i >= r || qb >= ldq) continue;
+      const std::size_t off = (static_cast<std::size_t>(mu) * r + i) * ldq + qb + 4 * (lane >> 4);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) split_bf16(acc[a][b][v], Uh[off + v], Ul[off + v]);
+      for (int v = 0; v < 4; ++v) split_bf16(acc[x][y][v], Uh[off + v], Ul[off + v]);
     }
 }
 
@@ -512,18 +564,25 @@ __global__ void split_transpose_kernel(const double* __restrict__ C, int n, int 
   lo[e] = l;
 }
 
-template <int NI>
+template <int NIW>
 void launch_u(Precision prec, const void* B, const OccupiedOperands& ops, int ldq, void* U, hipStream_t s) {
-  const int nqb = ldq / 128;
-  dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (16 * NI));
   if (prec == Precision::FP64) {
-    hipLaunchKernelGGL((occupied_u_kernel<double, NI>), grid, dim3(256), 0, s, static_cast<const double*>(B),
-                       ops.c64, static_cast<double*>(U), ops.n, ldq, ops.ldc, ops.r, nqb);
+    constexpr int WQ = 4;
+    const int nqb = (ldq + 32 * WQ - 1) / (32 * WQ);
+    dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (32 * NIW));
+    hipLaunchKernelGGL((occupied_u_kernel<double, NIW, WQ>), grid, dim3(128 * WQ), 0, s,
+                       static_cast<const double*>(B), ops.c64, static_cast<double*>(U), ops.n, ldq, ops.ldc, ops.r,
+                       nqb);
   } else if (prec == Precision::FP32) {
-    hipLaunchKernelGGL((occupied_u_kernel<float, NI>), grid, dim3(256), 0, s, static_cast<const float*>(B),
+    constexpr int WQ = 8;
+    const int nqb = (ldq + 32 * WQ - 1) / (32 * WQ);
+    dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (32 * NIW));
+    hipLaunchKernelGGL((occupied_u_kernel<float, NIW, WQ>), grid, dim3(128 * WQ), 0, s, static_cast<const float*>(B),
                        ops.c32, static_cast<float*>(U), ops.n, ldq, ops.ldc, ops.r, nqb);
   } else {
-    hipLaunchKernelGGL((occupied_u_bf16x3_kernel<NI>), grid, dim3(256), 0, s, static_cast<const float*>(B),
+    const int nqb = (ldq + kUTileQ - 1) / kUTileQ;
+    dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (32 * NIW));
+    hipLaunchKernelGGL((occupied_u_bf16x3_kernel<NIW>), grid, dim3(1024), 0, s, static_cast<const float*>(B),
                        static_cast<const __bf16*>(ops.ct_hi), static_cast<const __bf16*>(ops.ct_lo),
                        static_cast<__bf16*>(U), static_cast<__bf16*>(U) + static_cast<std::size_t>(ops.n) * ops.r * ldq,
                        ops.n, ldq, ops.ldct, ops.r, nqb);
@@ -535,16 +594,9 @@ void launch_u(Precision prec, const void* B, const OccupiedOperands& ops, int ld
 }  // namespace
 
 int occupied_tile(int r) {
-  int best = 1;
-  std::size_t best_pad = round_up(r, 16);
-  for (int ni = 1; ni <= 8; ++ni) {
-    const std::size_t padded = round_up(r, 16 * ni);
-    if (padded <= best_pad) {
-      best = ni;
-      best_pad = padded;
-    }
-  }
-  return best;
+  // i-tiles of 16 per wave, two waves along i: one block covers 32 * NIW
+  // columns, all of r up to 320.
+  return std::max(1, std::min(10, (r + 31) / 32));
 }
 
 void pack_density(const double* D, int n, double* dpacked, hipStream_t s) {
@@ -623,6 +675,8 @@ void occupied_u(Precision prec, const void* B, const OccupiedOperands& ops, int 
     case 6: launch_u<6>(prec, B, ops, ldq, U, s); break;
     case 7: launch_u<7>(prec, B, ops, ldq, U, s); break;
     case 8: launch_u<8>(prec, B, ops, ldq, U, s); break;
+    case 9: launch_u<9>(prec, B, ops, ldq, U, s); break;
+    case 10: launch_u<10>(prec, B, ops, ldq, U, s); break;
     default: throw std::logic_error("occupied tile width out of range");
   }
 }

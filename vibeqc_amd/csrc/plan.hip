@@ -265,7 +265,7 @@ void DfPlan::ensure_factor(int r) {
   }
   r = std::max(r, 1);
   const int ni = occupied_tile(r);
-  const int ldc = static_cast<int>(round_up(r, 16 * ni));
+  const int ldc = static_cast<int>(round_up(r, 32 * ni));
   if (r <= factor_cap_ && ni == ni_ && ldc == ldc_) return;
   ni_ = ni;
   ldc_ = ldc;
