@@ -1,17 +1,25 @@
 """DF-JK Fock-build benchmark on one MI355X.
 
-Workload (one "step" = one complete J + K build at fixed density, inputs
-resident in HBM): all-trans C20H42, def2-SVP orbital basis (490 AO; the
-offline data has no carbon cc-pVDZ), auxiliary cc-pVDZ-JKFIT for H and the
-oxygen cc-pVDZ-JKFIT shells scaled by (6/8)^2 for C (2366 functions), 81
-occupied orbitals drawn N(0,1) from a seed and Lowdin-orthonormalised in the
-AO overlap.  The same plan is timed in FP64, FP32 and BF16X3 (split bfloat16,
-FP32 accumulation); `value` is the BF16X3 rate, the FP64 and FP32 rates and the
-max-abs J/K deviations from FP64 are reported beside it.  A second workload,
-(H2O)4 cc-pVDZ/cc-pVDZ-JKFIT, gives SCF seconds per molecule in FP64 and in
-mixed precision (BF16X3 iterations + FP64 refinement).
+One "step" is one complete J + K build at fixed density with every input
+resident in HBM (B, the occupied factor, the output buffers).
 
-Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu-baseline]
+Headline workload (`value`): the single-build C20H42 shape -- all-trans
+n-alkane, 490 AO, 81 occupied columns drawn N(0,1) from a seed and Lowdin-
+orthonormalised in the AO overlap.  The offline basis data has no carbon
+cc-pVDZ / cc-pVDZ-JKFIT, so the orbital basis is def2-SVP (the same 490-AO
+shape) and carbon's fitting set is the oxygen JKFIT shells scaled by (6/8)^2
+(2366 auxiliary functions).  FP64, FP32 and BF16X3 (split bfloat16, FP32
+accumulation) are timed on the same plan; `value` is the BF16X3 ("mixed")
+rate, with the J/K max-abs deviation of every precision from FP64 beside it.
+
+Also reported:
+  * water64: (H2O)64 at liquid density (12.4 A cube), cc-pVDZ (1536 AO) /
+    cc-pVDZ-JKFIT (7424), 320 occupied -- the large single system the
+    mixed/FP64 speedup target is stated on, with its own roofline and CPU sample;
+  * scf_water4: (H2O)4 cc-pVDZ SCF seconds per molecule in FP64 and in mixed
+    precision (BF16X3 iterations, then FP64 refinement) with the energy gap.
+
+Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu-baseline]
 """
 
 from __future__ import annotations
@@ -31,9 +39,26 @@ sys.path.insert(0, str(ROOT))
 METRIC = "DF-JK Fock builds/s & SCF s/molecule, mixed vs FP64, 1 GPU, ≤1 µEh error"
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
+PRECISIONS = ("fp64", "fp32", "bf16x3")
 
 
-def _alkane_plan(precision="fp64"):
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _water64_plan():
+    from vibeqc_amd import AOBasis, FockPlan, shells_for
+    from vibeqc_amd.geometry import water_cluster
+
+    atoms = water_cluster(64, 12.4, seed=4)
+    orb = AOBasis(atoms, shells_for("cc-pvdz", atoms))
+    aux = AOBasis(atoms, shells_for("cc-pvdz-jkfit", atoms))
+    t0 = time.perf_counter()
+    plan = FockPlan(orb, auxiliary=aux)
+    return plan, time.perf_counter() - t0, 320
+
+
+def _alkane_plan():
     from vibeqc_amd import AOBasis, FockPlan, shells_for
     from vibeqc_amd.basis import cc_pvdz_aux_for_carbon
     from vibeqc_amd.geometry import alkane
@@ -42,13 +67,164 @@ def _alkane_plan(precision="fp64"):
     orb = AOBasis(atoms, shells_for("def2-svp", atoms))
     aux = AOBasis(atoms, shells_for(cc_pvdz_aux_for_carbon(), atoms))
     t0 = time.perf_counter()
-    plan = FockPlan(orb, auxiliary=aux, precision=precision)
-    return plan, time.perf_counter() - t0
+    plan = FockPlan(orb, auxiliary=aux)
+    return plan, time.perf_counter() - t0, 81
 
 
-def _cpu_baseline(n, naux, nocc, seed=7):
+def _time_precisions(plan, nocc, steps, warmup, precisions=PRECISIONS):
+    from vibeqc_amd.geometry import lowdin_occupied
+
+    S, _ = plan.one_electron()
+    C = lowdin_occupied(S, nocc, seed=1)
+    plan.set_precision("fp64")
+    j64, k64 = plan.evaluate_occupied(C)
+    out = {}
+    for precision in precisions:
+        plan.set_precision(precision)
+        plan.benchmark_occupied(C, repeat=max(1, warmup))
+        ms = plan.benchmark_occupied(C, repeat=steps)
+        j, k = plan.evaluate_occupied(C)
+        out[precision] = {
+            "builds_per_s": 1e3 / ms["wall_ms"],
+            "ms_per_build_wall": ms["wall_ms"],
+            "phase_ms": ms,
+            "max_abs_err_J": float(np.abs(j - j64).max()),
+            "max_abs_err_K": float(np.abs(k - k64).max()),
+            "rel_err_J": float(np.abs(j - j64).max() / np.abs(j64).max()),
+            "rel_err_K": float(np.abs(k - k64).max() / np.abs(k64).max()),
+        }
+        _log(f"{precision}: {ms['wall_ms']:.3f} ms/build, phases {ms}")
+    plan.set_precision("fp64")
+    return out
+
+
+# Device symbols of the kernels each timed phase launches (the phase times are
+# HIP events on the plan's stream around exactly these launches).
+PHASE_KERNELS = {
+    "exchange_u_ms": ("occupied_u", {"fp64": "occupied_u_kernel<double", "fp32": "occupied_u_kernel<float",
+                                     "bf16x3": "occupied_u_bf16x3_kernel"}),
+    "exchange_syrk_ms": ("exchange_syrk", {"fp64": "exchange_syrk_kernel<double", "fp32": "exchange_syrk_kernel<float",
+                                           "bf16x3": "exchange_syrk_bf16x3_kernel"}),
+    "coulomb_ms": ("coulomb_matrix", {"fp64": "coulomb_matrix_kernel<double", "fp32": "coulomb_matrix_kernel<float",
+                                      "bf16x3": "coulomb_matrix_kernel<float"}),
+}
+
+
+def _phase_work(n, naux, npair, nocc, precision):
+    """Algorithmic flops and HBM bytes of one launch of each phase.
+
+    U = B C: 2 n^2 A o flops; bytes = packed B once (P A s_B) + U written
+    (n o A s_U).  K = U U^T as a lower-triangle SYRK: n (n + 1) A o flops;
+    bytes = U read once + K written (n^2 8).  J = B rho: 2 P A flops; bytes =
+    packed B once.  s_B = 8 (FP64) or 4 (FP32 copy); s_U = 8, 4 or 2 + 2.
+    """
+    s_b = 8 if precision == "fp64" else 4
+    s_u = 8 if precision == "fp64" else 4
+    u_bytes = float(n) * nocc * naux * s_u
+    return {
+        "exchange_u_ms": (2.0 * n * n * naux * nocc, float(npair) * naux * s_b + u_bytes),
+        "exchange_syrk_ms": (float(n) * (n + 1) * naux * nocc, u_bytes + 8.0 * n * n),
+        "coulomb_ms": (2.0 * npair * naux, float(npair) * naux * s_b),
+    }
+
+
+def _roofline(phase_ms, n, naux, npair, nocc, precision, traffic=None):
+    """Roofline of the dominant (longest) kernel of one build.
+
+    The bound is whichever resource needs more time for the kernel's
+    algorithmic work: MFMA at the precision's dense peak (BF16X3 executes three
+    bf16 MFMAs per logical product, so its effective peak is 2.5 PFLOP/s / 3)
+    or HBM at 8 TB/s.  `achieved` = algorithmic work / HIP-event kernel time.
+    """
+    work = _phase_work(n, naux, npair, nocc, precision)
+    key = max(PHASE_KERNELS, key=lambda k: phase_ms[k])
+    label, symbols = PHASE_KERNELS[key]
+    flops, nbytes = work[key]
+    t = phase_ms[key] * 1e-3
+    peak_f = PEAK_TFLOPS[precision] / (3.0 if precision == "bf16x3" else 1.0)
+    t_f, t_b = flops / (peak_f * 1e12), nbytes / (HBM_PEAK_GBS * 1e9)
+    if t_f >= t_b:
+        bound, achieved, peak, unit = "mfma", flops / t / 1e12, peak_f, "TFLOP/s"
+    else:
+        bound, achieved, peak, unit = "hbm", nbytes / t / 1e9, HBM_PEAK_GBS, "GB/s"
+    sym = symbols[precision]
+    measured = (traffic or {}).get(sym)
+    return {
+        "kernel": sym,
+        "phase": label,
+        "bound": bound,
+        "achieved": achieved,
+        "peak": peak,
+        "unit": unit,
+        "frac": achieved / peak,
+        "traffic": None if measured is None else measured["bytes"],
+        "traffic_detail": measured if measured is not None else (traffic or {}).get("error"),
+        "algorithmic_flops": flops,
+        "algorithmic_bytes": nbytes,
+        "kernel_ms": phase_ms[key],
+    }
+
+
+def _pmc_traffic(workload, precision, copy_mib=256, timeout=240):
+    """HBM bytes per launch of this workload's kernels from two rocprofv3 PMC
+    runs (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) of a short child
+    process, each scaled by what the same counter reads for a device-to-device
+    copy of `copy_mib` MiB run first in the same child.  Returns
+    {kernel symbol fragment: bytes} for the symbols in PHASE_KERNELS, or an
+    {"error": ...} dict."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return {"error": "rocprofv3 not found"}
+    per_counter = {}
+    scale = {}
+    env = dict(os.environ, TMPDIR="/tmp")
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        out = tempfile.mkdtemp(prefix="vqc_pmc_", dir="/tmp")
+        cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "run", "--",
+               sys.executable, str(ROOT / "bench.py"), "--only", workload, "--precision", precision,
+               "--steps", "2", "--warmup", "1", "--copy-mib", str(copy_mib)]
+        try:
+            proc = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=timeout)
+        except subprocess.TimeoutExpired:
+            return {"error": f"rocprofv3 --pmc {counter} timed out"}
+        files = list(Path(out).rglob("run_counter_collection.csv"))
+        if proc.returncode != 0 or not files:
+            return {"error": f"rocprofv3 --pmc {counter} exit {proc.returncode}: {proc.stderr[-300:]}"}
+        rows = list(csv.DictReader(open(files[0])))
+        shutil.rmtree(out, ignore_errors=True)
+        rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+        # The calibration copy is the first blit after the fill of its source.
+        fill = next(i for i, r in enumerate(rows) if "FillFunctor" in r["Kernel_Name"])
+        copy = next(r for r in rows[fill + 1:] if "copyBuffer" in r["Kernel_Name"])
+        scale[counter] = copy_mib * 1024.0 / float(copy["Counter_Value"])
+        vals = {}
+        for _, symbols in PHASE_KERNELS.values():
+            sym = symbols[precision]
+            v = [float(r["Counter_Value"]) for r in rows if sym in r["Kernel_Name"]]
+            if v:
+                vals[sym] = sum(v) / len(v) * 1024.0 * scale[counter]
+        per_counter[counter] = vals
+    traffic = {}
+    for sym in per_counter["FETCH_SIZE"]:
+        traffic[sym] = {
+            "bytes": per_counter["FETCH_SIZE"][sym] + per_counter["WRITE_SIZE"].get(sym, 0.0),
+            "fetch_bytes": per_counter["FETCH_SIZE"][sym],
+            "write_bytes": per_counter["WRITE_SIZE"].get(sym, 0.0),
+            "fetch_scale": scale["FETCH_SIZE"],
+            "write_scale": scale["WRITE_SIZE"],
+        }
+    return traffic
+
+
+def _cpu_baseline(n, naux, nocc, fraction, seed=7):
     """The FP64 oracle's occupied-factor J/K (numpy/BLAS) on a bounded sample:
-    synthetic B of shape (n, n, naux/4), scaled x4 (the build is linear in naux)."""
+    synthetic B of shape (n, n, naux*fraction), time scaled by 1/fraction (the
+    build is linear in naux)."""
     from oracle import oracle as O
 
     try:
@@ -57,14 +233,14 @@ def _cpu_baseline(n, naux, nocc, seed=7):
         threads = max((p.get("num_threads", 1) for p in threadpool_info() if p.get("user_api") == "blas"), default=1)
     except Exception:
         threads = os.cpu_count() or 1
-    sub = max(1, naux // 4)
+    sub = max(1, int(naux * fraction))
     rng = np.random.default_rng(seed)
     B = rng.standard_normal((n, n, sub))
-    B = 0.5 * (B + B.transpose(1, 0, 2))
+    B += B.transpose(1, 0, 2)
     C = rng.standard_normal((n, nocc))
     O.jk_occupied(B, C)  # prime
     runs = []
-    for _ in range(3):
+    for _ in range(5):
         t0 = time.perf_counter()
         O.jk_occupied(B, C)
         runs.append(time.perf_counter() - t0)
@@ -74,7 +250,10 @@ def _cpu_baseline(n, naux, nocc, seed=7):
         "unit": "Fock builds/s",
         "cores": int(threads),
         "kind": "port",
-        "sample": f"oracle jk_occupied (numpy FP64) on synthetic B {n}x{n}x{sub} (naux/4), C {n}x{nocc}; median of 3 after a prime, scaled x{naux / sub:.2f} to naux={naux}",
+        "sample": (
+            f"oracle jk_occupied (numpy FP64, OpenBLAS) on synthetic B {n}x{n}x{sub} and C {n}x{nocc}: "
+            f"median of 5 after a prime, scaled x{naux / sub:.2f} to naux={naux}"
+        ),
     }
 
 
@@ -90,15 +269,48 @@ def _scf_timing(precision):
         t1 = time.perf_counter()
         r = plan.solve(energy_tolerance=1e-10, density_tolerance=1e-8)
         t2 = time.perf_counter()
-    return {"energy": r.energy, "iterations": r.iterations, "setup_s": t1 - t0, "scf_s": t2 - t1, "total_s": t2 - t0}
+    return {"energy": r.energy, "iterations": r.iterations, "fock_builds": r.fock_builds, "setup_s": t1 - t0,
+            "scf_s": t2 - t1, "total_s": t2 - t0}
+
+
+def _profile_run(args):
+    if args.copy_mib > 0:
+        import torch
+
+        count = args.copy_mib * (1 << 20) // 4
+        src = torch.ones(count, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        _log(f"copied {args.copy_mib} MiB device-to-device 3 times")
+    plan, setup, nocc = _alkane_plan() if args.only == "c20h42" else _water64_plan()
+    precisions = (args.precision,) if args.precision else PRECISIONS
+    res = _time_precisions(plan, nocc, args.steps, args.warmup, precisions)
+    plan.close()
+    print(json.dumps({"workload": args.only, "setup_s": setup, "results": res}))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 PMC byte-count runs")
+    ap.add_argument(
+        "--only",
+        choices=("c20h42", "water64"),
+        help="profiling run: time one workload only (no SCF, no CPU baseline) and print its phase times",
+    )
+    ap.add_argument("--precision", choices=PRECISIONS, help="with --only: time this precision only")
+    ap.add_argument(
+        "--copy-mib",
+        type=int,
+        default=0,
+        help="with --only: first run three device-to-device copies of this many MiB "
+        "(a known byte count to calibrate hardware byte counters against)",
+    )
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -107,83 +319,84 @@ def main():
             file=sys.stderr,
         )
         sys.exit(2)
+    if args.only:
+        _profile_run(args)
+        return
 
-    from vibeqc_amd.geometry import lowdin_occupied
+    aplan, asetup, anocc = _alkane_plan()
+    ashape = aplan.shape()
+    an, anaux, anpair = ashape["nbf"], ashape["naux"], ashape["npair"]
+    _log(f"C20H42 plan: nbf {an}, naux {anaux}, setup {asetup:.2f} s")
+    alk = _time_precisions(aplan, anocc, args.steps, args.warmup)
+    aplan.close()
 
-    plan, setup_s = _alkane_plan()
-    shape = plan.shape()
-    n, naux, npair = shape["nbf"], shape["naux"], shape["npair"]
-    nocc = 81
-    S, _ = plan.one_electron()
-    C = lowdin_occupied(S, nocc, seed=1)
-    j64, k64 = plan.evaluate_occupied(C)
+    wplan, wsetup, wnocc = _water64_plan()
+    wshape = wplan.shape()
+    wn, wnaux, wnpair = wshape["nbf"], wshape["naux"], wshape["npair"]
+    _log(f"(H2O)64 plan: nbf {wn}, naux {wnaux}, setup {wsetup:.2f} s")
+    w64 = _time_precisions(wplan, wnocc, args.steps, args.warmup)
+    wplan.close()
 
-    results = {}
-    for precision in ("fp64", "fp32", "bf16x3"):
-        plan.set_precision(precision)
-        plan.benchmark_occupied(C, repeat=max(1, args.warmup))
-        ms = plan.benchmark_occupied(C, repeat=args.steps)
-        wall = ms["wall_ms"] * 1e-3
-        j, k = plan.evaluate_occupied(C)
-        results[precision] = {
-            "builds_per_s": 1.0 / wall,
-            "ms_per_build_wall": wall * 1e3,
-            "phase_ms": ms,
-            "max_abs_err_J": float(np.abs(j - j64).max()),
-            "max_abs_err_K": float(np.abs(k - k64).max()),
-        }
-        print(f"[bench] {precision}: {1e3 * wall:.3f} ms/build, phases {ms}", file=sys.stderr, flush=True)
-
-    head = results["bf16x3"]
-    ph = head["phase_ms"]
-    gemm_flops = 2.0 * n * n * naux * nocc
-    # Dominant kernel of the headline build and its algorithmic work.
-    phases = {
-        "exchange_u": (ph["exchange_u_ms"], "mfma", gemm_flops / (ph["exchange_u_ms"] * 1e-3) / 1e12, PEAK_TFLOPS["bf16x3"], "TFLOP/s"),
-        "exchange_syrk": (ph["exchange_syrk_ms"], "mfma", gemm_flops / (ph["exchange_syrk_ms"] * 1e-3) / 1e12, PEAK_TFLOPS["bf16x3"], "TFLOP/s"),
-        "coulomb": (ph["coulomb_ms"], "hbm", 2.0 * npair * naux * 4 / (ph["coulomb_ms"] * 1e-3) / 1e9, HBM_PEAK_GBS, "GB/s"),
-    }
-    name, (t_ms, bound, achieved, peak, unit) = max(phases.items(), key=lambda kv: kv[1][0])
-    roofline = {
-        "kernel": name,
-        "bound": bound,
-        "achieved": achieved,
-        "peak": peak,
-        "unit": unit,
-        "frac": achieved / peak,
-        "traffic": None,
-    }
-    cpu = None if args.no_cpu_baseline else _cpu_baseline(n, naux, nocc)
+    cpu = cpu64 = None
+    if not args.no_cpu_baseline:
+        cpu = _cpu_baseline(an, anaux, anocc, fraction=0.5)
+        _log(f"cpu baseline C20H42: {cpu}")
+        cpu64 = _cpu_baseline(wn, wnaux, wnocc, fraction=1.0 / 32.0)
+        _log(f"cpu baseline (H2O)64: {cpu64}")
     scf64 = _scf_timing("fp64")
     scfmx = _scf_timing("bf16x3")
-    plan.close()
+    traffic = traffic64 = None
+    if not args.no_traffic:
+        traffic = _pmc_traffic("c20h42", "bf16x3")
+        _log(f"PMC traffic C20H42: {traffic}")
+        traffic64 = _pmc_traffic("water64", "bf16x3")
+        _log(f"PMC traffic (H2O)64: {traffic64}")
     line = {
         "metric": METRIC,
-        "value": head["builds_per_s"],
+        "value": alk["bf16x3"]["builds_per_s"],
         "unit": "Fock builds/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": head["ms_per_build_wall"],
+        "ms_per_step": alk["bf16x3"]["ms_per_build_wall"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16x3",
         "data": "synthetic",
         "config": {
-            "workload": "C20H42 def2-SVP (490 AO) / jkfit-style aux (2366), DF-JK Fock build at fixed density, 81 occupied",
-            "nbf": n,
-            "naux": naux,
-            "nocc": nocc,
+            "workload": "C20H42 single DF-JK Fock build at fixed density, def2-SVP (490 AO) / "
+            "JKFIT-shaped auxiliary (2366), 81 occupied",
+            "nbf": an,
+            "naux": anaux,
+            "nocc": anocc,
             "parallelism": "single-gpu",
-            "setup_s": setup_s,
+            "setup_s": asetup,
         },
-        "fp64_builds_per_s": results["fp64"]["builds_per_s"],
-        "fp32_builds_per_s": results["fp32"]["builds_per_s"],
-        "mixed_speedup_vs_fp64": head["builds_per_s"] / results["fp64"]["builds_per_s"],
-        "precision_results": results,
-        "scf_water4": {"fp64": scf64, "mixed": scfmx, "energy_diff_mixed_minus_fp64": scfmx["energy"] - scf64["energy"]},
-        "roofline": roofline,
+        "fp64_builds_per_s": alk["fp64"]["builds_per_s"],
+        "fp32_builds_per_s": alk["fp32"]["builds_per_s"],
+        "mixed_speedup_vs_fp64": alk["bf16x3"]["builds_per_s"] / alk["fp64"]["builds_per_s"],
+        "precision_results": alk,
+        "roofline": _roofline(alk["bf16x3"]["phase_ms"], an, anaux, anpair, anocc, "bf16x3", traffic),
+        "water64": {
+            "workload": "(H2O)64 cc-pVDZ / cc-pVDZ-JKFIT, DF-JK Fock build at fixed density, 320 occupied",
+            "nbf": wn,
+            "naux": wnaux,
+            "nocc": wnocc,
+            "setup_s": wsetup,
+            "mixed_builds_per_s": w64["bf16x3"]["builds_per_s"],
+            "fp64_builds_per_s": w64["fp64"]["builds_per_s"],
+            "mixed_speedup_vs_fp64": w64["bf16x3"]["builds_per_s"] / w64["fp64"]["builds_per_s"],
+            "precision_results": w64,
+            "roofline": _roofline(w64["bf16x3"]["phase_ms"], wn, wnaux, wnpair, wnocc, "bf16x3", traffic64),
+            "roofline_fp64": _roofline(w64["fp64"]["phase_ms"], wn, wnaux, wnpair, wnocc, "fp64"),
+            "cpu_baseline": cpu64,
+        },
+        "scf_water4": {
+            "fp64": scf64,
+            "mixed": scfmx,
+            "energy_diff_mixed_minus_fp64": scfmx["energy"] - scf64["energy"],
+        },
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -177,7 +177,7 @@ GENERATIVEQC_API generativeqc_status vibeqc_fock_plan_evaluate_occupied(
  * untimed build, synchronises, then runs `repeat` complete J+K builds on the
  * device with no host transfer in between and synchronises again.
  * kernel_ms (may be NULL, else 5 entries) receives per-build averages:
- * device times from HIP events on the plan's stream, [0] U = B C,
+ * device times from HIP events on the plan's stream, [0] U = B C (with the rho partials),
  * [1] K = U U^T, [2] J = B rho, [3] whole build, and [4] the host wall time
  * between the two synchronisations divided by `repeat`. */
 GENERATIVEQC_API generativeqc_status vibeqc_fock_plan_benchmark_occupied(

@@ -116,10 +116,30 @@ __global__ void sum_rows_kernel(const double* __restrict__ partial, int rows, in
   out[c] = s;
 }
 
-template <class T, int V>
-__global__ __launch_bounds__(256) void coulomb_matrix_kernel(const T* __restrict__ B, const double* __restrict__ rho,
-                                                            std::size_t npair, int ldq, int n,
-                                                            double* __restrict__ J) {
+template <class T>
+struct Vec16;
+template <>
+struct Vec16<double> {
+  using type = double __attribute__((ext_vector_type(2)));
+};
+template <>
+struct Vec16<float> {
+  using type = float __attribute__((ext_vector_type(4)));
+};
+
+constexpr int kJThreads = 1024, kJInFlight = 8;
+
+// J_p = sum_Q B[p][Q] rho_Q, one wavefront per packed row.  Each lane keeps
+// kJInFlight 16-byte loads of its row in flight before consuming them, and a
+// 1024-thread block shares one LDS copy of rho, so two blocks (32 waves) fit
+// on a CU: enough outstanding bytes to stream B at HBM rate.
+template <class T>
+__global__ __launch_bounds__(kJThreads) __attribute__((amdgpu_waves_per_eu(8))) void coulomb_matrix_kernel(const T* __restrict__ B,
+                                                                  const double* __restrict__ rho,
+                                                                  std::size_t npair, int ldq, int n,
+                                                                  double* __restrict__ J) {
+  using V = typename Vec16<T>::type;
+  constexpr int W = 16 / sizeof(T), STEP = 64 * W;
   extern __shared__ double srho[];
   for (int q = threadIdx.x; q < ldq; q += blockDim.x) srho[q] = rho[q];
   __syncthreads();
@@ -129,9 +149,21 @@ __global__ __launch_bounds__(256) void coulomb_matrix_kernel(const T* __restrict
   for (std::size_t p = wave; p < npair; p += nwave) {
     const T* row = B + p * ldq;
     double s = 0.0;
-    for (int q = lane * V; q < ldq; q += 64 * V) {
+    for (int q0 = lane * W; q0 < ldq; q0 += STEP * kJInFlight) {
+      V v[kJInFlight];
 #pragma unroll
-      for (int v = 0; v < V; ++v) s += static_cast<double>(row[q + v]) * srho[q + v];
+      for (int u = 0; u < kJInFlight; ++u) {
+        const int q = q0 + u * STEP;
+        v[u] = q < ldq ? *reinterpret_cast<const V*>(row + q) : V{};
+      }
+#pragma unroll
+      for (int u = 0; u < kJInFlight; ++u) {
+        const int q = q0 + u * STEP;
+        if (q < ldq) {
+#pragma unroll
+          for (int e = 0; e < W; ++e) s += static_cast<double>(v[u][e]) * srho[q + e];
+        }
+      }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
@@ -158,7 +190,8 @@ constexpr int kUTileQ = 256;
 template <class T, int NIW, int WQ>
 __global__ __launch_bounds__(128 * WQ) void occupied_u_kernel(const T* __restrict__ B, const T* __restrict__ C,
                                                              T* __restrict__ U, int n, int ldq, int ldc, int r,
-                                                             int nqb) {
+                                                             int nqb, const double* __restrict__ D,
+                                                             double* __restrict__ rho_part) {
   using Acc = typename AccOf<T>::type;
   constexpr int BQ = 32 * WQ, BK = 16, BI = 32 * NIW, NT = 128 * WQ;
   constexpr int CP = (BK * BI + NT - 1) / NT;  // C elements staged per thread
@@ -175,18 +208,30 @@ __global__ __launch_bounds__(128 * WQ) void occupied_u_kernel(const T* __restric
 #pragma unroll
     for (int y = 0; y < NIW; ++y) acc[x][y] = Acc{};
   const int bk = tid / (BQ / 4), bq = (tid % (BQ / 4)) * 4;
+  // The first i-block also forms rho_Q partials, sum_nu D[mu][nu] B[(mu,nu)][Q],
+  // from the B values it streams anyway (FP64 accumulation).
+  const bool with_rho = rho_part != nullptr && blockIdx.y == 0;
+  double racc[4] = {0.0, 0.0, 0.0, 0.0};
+  extern __shared__ double drow[];  // D[mu][:], staged once when rho is formed
+  if (with_rho)
+    for (int k = tid; k < n; k += NT) drow[k] = D[static_cast<std::size_t>(mu) * n + k];
+  __syncthreads();
   T rb[4], rc[CP];
+  double rd = 0.0;
   // Stage the next k-step in registers so its global latency overlaps the
   // MFMAs of the current one; LDS is double-buffered (one barrier per step).
+  // Staged values are only consumed in sstore, after the MFMAs.
   auto gload = [&](int nu0) {
     const int nu = nu0 + bk, q = q0 + bq;
     if (nu < n && q < ldq) {
       const T* src = B + packed_index(mu, nu) * ldq + q;
 #pragma unroll
       for (int e = 0; e < 4; ++e) rb[e] = src[e];
+      rd = nu;
     } else {
 #pragma unroll
       for (int e = 0; e < 4; ++e) rb[e] = T(0);
+      rd = -1.0;
     }
 #pragma unroll
     for (int j = 0; j < CP; ++j) {
@@ -198,6 +243,11 @@ __global__ __launch_bounds__(128 * WQ) void occupied_u_kernel(const T* __restric
   auto sstore = [&](int buf) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) Bs[buf][bk][bq + e] = rb[e];
+    if (with_rho) {
+      const double d = rd >= 0.0 ? drow[static_cast<int>(rd)] : 0.0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) racc[e] += d * static_cast<double>(rb[e]);
+    }
 #pragma unroll
     for (int j = 0; j < CP; ++j) {
       const int e = tid + j * NT;
@@ -227,6 +277,19 @@ __global__ __launch_bounds__(128 * WQ) void occupied_u_kernel(const T* __restric
     __syncthreads();
     cur ^= 1;
   }
+  if (with_rho) {
+    // Reduce the BK row groups of each Q column through LDS (the B tiles are free).
+    double* red = reinterpret_cast<double*>(&Bs[0][0][0]);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) red[bk * BQ + bq + e] = racc[e];
+    __syncthreads();
+    if (tid < BQ && q0 + tid < ldq) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < BK; ++k) t += red[k * BQ + tid];
+      rho_part[static_cast<std::size_t>(mu) * ldq + q0 + tid] = t;
+    }
+  }
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -251,7 +314,10 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
                                                                 const __bf16* __restrict__ Cth,
                                                                 const __bf16* __restrict__ Ctl,
                                                                 __bf16* __restrict__ Uh, __bf16* __restrict__ Ul,
-                                                                int n, int ldq, int ldct, int r, int nqb) {
+                                                                int n, int ldq, int ldct, int r, int nqb,
+                                                                const double* __restrict__ D,
+                                                                double* __restrict__ rho_part) {
+  // 16 waves: 8 along Q (32 rows each) x 2 along i (16*NIW columns each).
   constexpr int BQ = kUTileQ, BK = 32, BI = 32 * NIW, PAD = 8, NT = 1024;
   constexpr int YP = (BI * 4 + NT - 1) / NT;  // 16-byte C chunks per thread and plane
   __shared__ __bf16 Xh[BQ][BK + PAD], Xl[BQ][BK + PAD];
@@ -268,8 +334,17 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
     for (int y = 0; y < NIW; ++y) acc[x][y] = f4{};
   const int xq = tid % BQ, xg = tid / BQ;  // one Q column, 8 consecutive nu
   float rx[8];
+  const bool with_rho = rho_part != nullptr && blockIdx.y == 0;
+  double racc = 0.0;
+  int rnu0 = 0;
+  extern __shared__ double drow[];  // D[mu][:], staged once when rho is formed
+  if (with_rho)
+    for (int k = tid; k < n; k += NT) drow[k] = D[static_cast<std::size_t>(mu) * n + k];
+  __syncthreads();
   // Only the B gather (HBM) is staged in registers; the small C^T tile is
-  // L2-resident and is copied straight into LDS.
+  // L2-resident and is copied straight into LDS.  The first i-block also
+  // accumulates rho_Q partials from the staged B values (FP32 B, FP64 sums)
+  // when they are consumed in sstore, so the loads stay in flight.
   auto gload = [&](int nu0) {
     const int q = q0 + xq;
 #pragma unroll
@@ -277,8 +352,16 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
       const int nu = nu0 + xg * 8 + e;
       rx[e] = (nu < n && q < ldq) ? B[packed_index(mu, nu) * ldq + q] : 0.0f;
     }
+    rnu0 = nu0;
   };
   auto sstore = [&](int nu0) {
+    if (with_rho) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int nu = rnu0 + xg * 8 + e;
+        if (nu < n) racc += drow[nu] * static_cast<double>(rx[e]);
+      }
+    }
     bf8 h, l;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
@@ -315,17 +398,29 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
       const int row = wi * 16 * NIW + y * 16 + (lane & 15);
       const bf8 bh = *reinterpret_cast<const bf8*>(&Yh[row][kq]);
       const bf8 bl = *reinterpret_cast<const bf8*>(&Yl[row][kq]);
+      // Alternate the two accumulators so no MFMA waits on its predecessor.
       acc[0][y] = mfma(ah0, bh, acc[0][y]);
-      acc[0][y] = mfma(ah0, bl, acc[0][y]);
-      acc[0][y] = mfma(al0, bh, acc[0][y]);
       acc[1][y] = mfma(ah1, bh, acc[1][y]);
+      acc[0][y] = mfma(ah0, bl, acc[0][y]);
       acc[1][y] = mfma(ah1, bl, acc[1][y]);
+      acc[0][y] = mfma(al0, bh, acc[0][y]);
       acc[1][y] = mfma(al1, bh, acc[1][y]);
     }
     __syncthreads();
     if (more) {
       sstore(nu0 + BK);
       __syncthreads();
+    }
+  }
+  if (with_rho) {
+    double* red = reinterpret_cast<double*>(&Xh[0][0]);  // NT doubles fit in Xh
+    red[xg * BQ + xq] = racc;
+    __syncthreads();
+    if (tid < BQ && q0 + tid < ldq) {
+      double t = 0.0;
+#pragma unroll
+      for (int g = 0; g < NT / BQ; ++g) t += red[g * BQ + tid];
+      rho_part[static_cast<std::size_t>(mu) * ldq + q0 + tid] = t;
     }
   }
 #pragma unroll
@@ -364,13 +459,16 @@ __global__ __launch_bounds__(512) void exchange_syrk_kernel(const T* __restrict_
 // triangle).  Block ids are remapped so the blocks resident on one XCD
 // (hardware dispatch is round-robin over the 8 XCDs) work on the same k-range
 // of W and share that XCD's L2.
-constexpr int kSyrkTM = 256, kSyrkTN = 128;
+constexpr int kSyrkTM = 256, kSyrkTN = 128;  // FP64 / FP32 tiles
+constexpr int kSyrkBfT = 256;                 // BF16X3 tiles (square)
 
-__device__ __forceinline__ void syrk_tile_of(int t, int ncb, int& rb, int& cb) {
+// Lower-triangle tile t -> (row block, column block); row block rb holds
+// min(ratio (rb + 1), ncb) column blocks, ratio = TM / TN.
+__device__ __forceinline__ void syrk_tile_of(int t, int ncb, int ratio, int& rb, int& cb) {
   int base = 0;
   rb = 0;
   for (;;) {
-    const int cnt = min(2 * (rb + 1), ncb);
+    const int cnt = min(ratio * (rb + 1), ncb);
     if (t < base + cnt) {
       cb = t - base;
       return;
@@ -380,85 +478,106 @@ __device__ __forceinline__ void syrk_tile_of(int t, int ncb, int& rb, int& cb) {
   }
 }
 
+// BF16X3 SYRK: 256 x 256 tiles, 8 waves as 2 (rows) x 4 (columns), each wave
+// 128 x 64 = 8 x 4 MFMA 16x16x32 tiles.  Per k-step of 32 a wave reads 12
+// fragment pairs from LDS for 96 MFMAs.  The next k-step is staged in
+// registers while the current one runs (one LDS buffer, two barriers).
 __global__ __launch_bounds__(512) void exchange_syrk_bf16x3_kernel(const __bf16* __restrict__ Wh,
                                                                   const __bf16* __restrict__ Wl, std::size_t kd,
                                                                   int n, int ntiles, int ncb, int nwork,
                                                                   std::size_t kbegin, std::size_t kend,
                                                                   std::size_t kchunk, double sign,
                                                                   double* __restrict__ partial) {
-  constexpr int TM = kSyrkTM, TN = kSyrkTN, BK = 32, PAD = 8, NT = 512;
-  __shared__ __bf16 Ah[TM][BK + PAD], Al[TM][BK + PAD];
-  __shared__ __bf16 Bh[TN][BK + PAD], Bl[TN][BK + PAD];
+  constexpr int T = kSyrkBfT, BK = 32, PAD = 8, NT = 512, XF = 8, YF = 4;
+  __shared__ __bf16 Ah[T][BK + PAD], Al[T][BK + PAD];
+  __shared__ __bf16 Bh[T][BK + PAD], Bl[T][BK + PAD];
   const int per = gridDim.x / 8;
   const int work = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (work >= nwork) return;
   const int split = work / ntiles, tile = work % ntiles;
   int rb, cb;
-  syrk_tile_of(tile, ncb, rb, cb);
-  const int m0 = rb * TM, l0 = cb * TN;
+  syrk_tile_of(tile, ncb, 1, rb, cb);
+  const int m0 = rb * T, l0 = cb * T;
   const std::size_t k0 = kbegin + static_cast<std::size_t>(split) * kchunk;
   const std::size_t k1 = k0 + kchunk < kend ? k0 + kchunk : kend;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
-  f4 acc[4][4];
+  const int wr = wave >> 2, wc = wave & 3;
+  f4 acc[XF][YF];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < XF; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = f4{};
+    for (int j = 0; j < YF; ++j) acc[i][j] = f4{};
   const bf8 zero = bf8{};
+  constexpr int CH = T * 4 / NT;  // 16-byte chunks per thread, plane and operand
+  bf8 ra[CH][2], rbv[CH][2];
+  auto gload = [&](std::size_t kb) {
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      const int c = tid + j * NT, row = c >> 2, kc = (c & 3) << 3;
+      const int ga = m0 + row, gb = l0 + row;
+      ra[j][0] = ra[j][1] = rbv[j][0] = rbv[j][1] = zero;
+      if (ga < n) {
+        const std::size_t off = static_cast<std::size_t>(ga) * kd + kb + kc;
+        ra[j][0] = *reinterpret_cast<const bf8*>(Wh + off);
+        ra[j][1] = *reinterpret_cast<const bf8*>(Wl + off);
+      }
+      if (gb < n) {
+        const std::size_t off = static_cast<std::size_t>(gb) * kd + kb + kc;
+        rbv[j][0] = *reinterpret_cast<const bf8*>(Wh + off);
+        rbv[j][1] = *reinterpret_cast<const bf8*>(Wl + off);
+      }
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      const int c = tid + j * NT, row = c >> 2, kc = (c & 3) << 3;
+      *reinterpret_cast<bf8*>(&Ah[row][kc]) = ra[j][0];
+      *reinterpret_cast<bf8*>(&Al[row][kc]) = ra[j][1];
+      *reinterpret_cast<bf8*>(&Bh[row][kc]) = rbv[j][0];
+      *reinterpret_cast<bf8*>(&Bl[row][kc]) = rbv[j][1];
+    }
+  };
+  gload(k0);
+  sstore();
+  __syncthreads();
   for (std::size_t kb = k0; kb < k1; kb += BK) {
-    for (int c = tid; c < TM * 4; c += NT) {
-      const int row = c >> 2, kc = (c & 3) << 3, grow = m0 + row;
-      bf8 h = zero, l = zero;
-      if (grow < n) {
-        const std::size_t off = static_cast<std::size_t>(grow) * kd + kb + kc;
-        h = *reinterpret_cast<const bf8*>(Wh + off);
-        l = *reinterpret_cast<const bf8*>(Wl + off);
-      }
-      *reinterpret_cast<bf8*>(&Ah[row][kc]) = h;
-      *reinterpret_cast<bf8*>(&Al[row][kc]) = l;
-    }
-    for (int c = tid; c < TN * 4; c += NT) {
-      const int row = c >> 2, kc = (c & 3) << 3, grow = l0 + row;
-      bf8 h = zero, l = zero;
-      if (grow < n) {
-        const std::size_t off = static_cast<std::size_t>(grow) * kd + kb + kc;
-        h = *reinterpret_cast<const bf8*>(Wh + off);
-        l = *reinterpret_cast<const bf8*>(Wl + off);
-      }
-      *reinterpret_cast<bf8*>(&Bh[row][kc]) = h;
-      *reinterpret_cast<bf8*>(&Bl[row][kc]) = l;
-    }
-    __syncthreads();
+    const bool more = kb + BK < k1;
+    if (more) gload(kb + BK);
     const int kq = 8 * (lane >> 4);
-    bf8 bh[4], bl[4];
+    bf8 bh[YF], bl[YF];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      bh[j] = *reinterpret_cast<const bf8*>(&Bh[wc * 64 + j * 16 + (lane & 15)][kq]);
-      bl[j] = *reinterpret_cast<const bf8*>(&Bl[wc * 64 + j * 16 + (lane & 15)][kq]);
+    for (int j = 0; j < YF; ++j) {
+      bh[j] = *reinterpret_cast<const bf8*>(&Bh[wc * 16 * YF + j * 16 + (lane & 15)][kq]);
+      bl[j] = *reinterpret_cast<const bf8*>(&Bl[wc * 16 * YF + j * 16 + (lane & 15)][kq]);
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const bf8 ah = *reinterpret_cast<const bf8*>(&Ah[wr * 64 + i * 16 + (lane & 15)][kq]);
-      const bf8 al = *reinterpret_cast<const bf8*>(&Al[wr * 64 + i * 16 + (lane & 15)][kq]);
+    for (int i = 0; i < XF; ++i) {
+      const bf8 ah = *reinterpret_cast<const bf8*>(&Ah[wr * 16 * XF + i * 16 + (lane & 15)][kq]);
+      const bf8 al = *reinterpret_cast<const bf8*>(&Al[wr * 16 * XF + i * 16 + (lane & 15)][kq]);
+      // Independent accumulators between dependent MFMAs keep the pipe full.
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        acc[i][j] = mfma(ah, bh[j], acc[i][j]);
-        acc[i][j] = mfma(ah, bl[j], acc[i][j]);
-        acc[i][j] = mfma(al, bh[j], acc[i][j]);
-      }
+      for (int j = 0; j < YF; ++j) acc[i][j] = mfma(ah, bh[j], acc[i][j]);
+#pragma unroll
+      for (int j = 0; j < YF; ++j) acc[i][j] = mfma(ah, bl[j], acc[i][j]);
+#pragma unroll
+      for (int j = 0; j < YF; ++j) acc[i][j] = mfma(al, bh[j], acc[i][j]);
     }
     __syncthreads();
+    if (more) {
+      sstore();
+      __syncthreads();
+    }
   }
   double* out = partial + static_cast<std::size_t>(split) * n * n;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < XF; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int lam = l0 + wc * 64 + j * 16 + (lane & 15);
+    for (int j = 0; j < YF; ++j) {
+      const int lam = l0 + wc * 16 * YF + j * 16 + (lane & 15);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int mu = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + v;
+        const int mu = m0 + wr * 16 * XF + i * 16 + 4 * (lane >> 4) + v;
         if (mu < n && lam < n && lam <= mu)
           out[static_cast<std::size_t>(mu) * n + lam] = sign * static_cast<double>(acc[i][j][v]);
       }
@@ -472,14 +591,14 @@ __global__ __launch_bounds__(512) void exchange_syrk_kernel(const T* __restrict_
                                                            double* __restrict__ partial) {
   using Acc = typename AccOf<T>::type;
   constexpr int TM = kSyrkTM, TN = kSyrkTN, BK = 16, PAD = 1, NT = 512, V = 16 / sizeof(T);
-  __shared__ T As[TM][BK + PAD];
-  __shared__ T Bs[TN][BK + PAD];
+  __shared__ T As[2][TM][BK + PAD];
+  __shared__ T Bs[2][TN][BK + PAD];
   const int per = gridDim.x / 8;
   const int work = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (work >= nwork) return;
   const int split = work / ntiles, tile = work % ntiles;
   int rb, cb;
-  syrk_tile_of(tile, ncb, rb, cb);
+  syrk_tile_of(tile, ncb, TM / TN, rb, cb);
   const int m0 = rb * TM, l0 = cb * TN;
   const std::size_t k0 = kbegin + static_cast<std::size_t>(split) * kchunk;
   const std::size_t k1 = k0 + kchunk < kend ? k0 + kchunk : kend;
@@ -490,33 +609,61 @@ __global__ __launch_bounds__(512) void exchange_syrk_kernel(const T* __restrict_
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = Acc{};
-  constexpr int CH = BK / V;  // 16-byte chunks per row
+  constexpr int CH = BK / V;                  // 16-byte chunks per row
+  constexpr int AC = TM * CH / NT, BC = TN * CH / NT;
+  T ra[AC][V], rbv[BC][V];
+  // Next k-step staged in registers, LDS double-buffered: one barrier per step.
+  auto gload = [&](std::size_t kb) {
+#pragma unroll
+    for (int j = 0; j < AC; ++j) {
+      const int c = tid + j * NT, row = c / CH, kc = (c % CH) * V, grow = m0 + row;
+#pragma unroll
+      for (int e = 0; e < V; ++e) ra[j][e] = grow < n ? W[static_cast<std::size_t>(grow) * kd + kb + kc + e] : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < BC; ++j) {
+      const int c = tid + j * NT, row = c / CH, kc = (c % CH) * V, grow = l0 + row;
+#pragma unroll
+      for (int e = 0; e < V; ++e) rbv[j][e] = grow < n ? W[static_cast<std::size_t>(grow) * kd + kb + kc + e] : T(0);
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < AC; ++j) {
+      const int c = tid + j * NT, row = c / CH, kc = (c % CH) * V;
+#pragma unroll
+      for (int e = 0; e < V; ++e) As[buf][row][kc + e] = ra[j][e];
+    }
+#pragma unroll
+    for (int j = 0; j < BC; ++j) {
+      const int c = tid + j * NT, row = c / CH, kc = (c % CH) * V;
+#pragma unroll
+      for (int e = 0; e < V; ++e) Bs[buf][row][kc + e] = rbv[j][e];
+    }
+  };
+  gload(k0);
+  sstore(0);
+  __syncthreads();
+  int cur = 0;
   for (std::size_t kb = k0; kb < k1; kb += BK) {
-    for (int c = tid; c < TM * CH; c += NT) {
-      const int row = c / CH, kc = (c % CH) * V, grow = m0 + row;
-#pragma unroll
-      for (int e = 0; e < V; ++e) As[row][kc + e] = grow < n ? W[static_cast<std::size_t>(grow) * kd + kb + kc + e] : T(0);
-    }
-    for (int c = tid; c < TN * CH; c += NT) {
-      const int row = c / CH, kc = (c % CH) * V, grow = l0 + row;
-#pragma unroll
-      for (int e = 0; e < V; ++e) Bs[row][kc + e] = grow < n ? W[static_cast<std::size_t>(grow) * kd + kb + kc + e] : T(0);
-    }
-    __syncthreads();
+    const bool more = kb + BK < k1;
+    if (more) gload(kb + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int k = kk + (lane >> 4);
       T b[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = Bs[wc * 64 + j * 16 + (lane & 15)][k];
+      for (int j = 0; j < 4; ++j) b[j] = Bs[cur][wc * 64 + j * 16 + (lane & 15)][k];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const T a = As[wr * 64 + i * 16 + (lane & 15)][k];
+        const T a = As[cur][wr * 64 + i * 16 + (lane & 15)][k];
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a, b[j], acc[i][j]);
       }
     }
+    if (more) sstore(cur ^ 1);
     __syncthreads();
+    cur ^= 1;
   }
   double* out = partial + static_cast<std::size_t>(split) * n * n;
 #pragma unroll
@@ -565,27 +712,30 @@ __global__ void split_transpose_kernel(const double* __restrict__ C, int n, int 
 }
 
 template <int NIW>
-void launch_u(Precision prec, const void* B, const OccupiedOperands& ops, int ldq, void* U, hipStream_t s) {
+void launch_u(Precision prec, const void* B, const OccupiedOperands& ops, int ldq, void* U, const double* D,
+              double* rho_part, hipStream_t s) {
+  const std::size_t dyn = rho_part ? static_cast<std::size_t>(ops.n) * sizeof(double) : 0;
+  if (dyn > 32 * 1024) throw NotImplemented("fused rho supports nbf <= 4096");
   if (prec == Precision::FP64) {
     constexpr int WQ = 4;
     const int nqb = (ldq + 32 * WQ - 1) / (32 * WQ);
     dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (32 * NIW));
-    hipLaunchKernelGGL((occupied_u_kernel<double, NIW, WQ>), grid, dim3(128 * WQ), 0, s,
+    hipLaunchKernelGGL((occupied_u_kernel<double, NIW, WQ>), grid, dim3(128 * WQ), dyn, s,
                        static_cast<const double*>(B), ops.c64, static_cast<double*>(U), ops.n, ldq, ops.ldc, ops.r,
-                       nqb);
+                       nqb, D, rho_part);
   } else if (prec == Precision::FP32) {
     constexpr int WQ = 8;
     const int nqb = (ldq + 32 * WQ - 1) / (32 * WQ);
     dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (32 * NIW));
-    hipLaunchKernelGGL((occupied_u_kernel<float, NIW, WQ>), grid, dim3(128 * WQ), 0, s, static_cast<const float*>(B),
-                       ops.c32, static_cast<float*>(U), ops.n, ldq, ops.ldc, ops.r, nqb);
+    hipLaunchKernelGGL((occupied_u_kernel<float, NIW, WQ>), grid, dim3(128 * WQ), dyn, s, static_cast<const float*>(B),
+                       ops.c32, static_cast<float*>(U), ops.n, ldq, ops.ldc, ops.r, nqb, D, rho_part);
   } else {
     const int nqb = (ldq + kUTileQ - 1) / kUTileQ;
     dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (32 * NIW));
-    hipLaunchKernelGGL((occupied_u_bf16x3_kernel<NIW>), grid, dim3(1024), 0, s, static_cast<const float*>(B),
+    hipLaunchKernelGGL((occupied_u_bf16x3_kernel<NIW>), grid, dim3(1024), dyn, s, static_cast<const float*>(B),
                        static_cast<const __bf16*>(ops.ct_hi), static_cast<const __bf16*>(ops.ct_lo),
                        static_cast<__bf16*>(U), static_cast<__bf16*>(U) + static_cast<std::size_t>(ops.n) * ops.r * ldq,
-                       ops.n, ldq, ops.ldct, ops.r, nqb);
+                       ops.n, ldq, ops.ldct, ops.r, nqb, D, rho_part);
   }
   VQC_LAUNCH("occupied_u_kernel");
 }
@@ -637,16 +787,24 @@ void coulomb_rho(Precision prec, const void* B, const double* dpacked, std::size
   VQC_LAUNCH("sum_rows_kernel");
 }
 
+void rho_from_partials(const double* rho_part, int rows, int ldq, double* rho, hipStream_t s) {
+  hipLaunchKernelGGL(sum_rows_kernel, dim3((ldq + 255) / 256), dim3(256), 0, s, rho_part, rows, ldq, rho);
+  VQC_LAUNCH("sum_rows_kernel");
+}
+
 void coulomb_matrix(Precision prec, const void* B, const double* rho, std::size_t npair, int ldq, int n, double* J,
                     hipStream_t s) {
   const std::size_t lds = static_cast<std::size_t>(ldq) * sizeof(double);
   if (lds > 160 * 1024) throw NotImplemented("auxiliary dimension exceeds the LDS-resident rho limit");
-  const unsigned blocks = static_cast<unsigned>(std::min<std::size_t>((npair + 3) / 4, 4096));
+  // Up to two resident blocks per CU on 256 CUs (one when rho exceeds 80 KB);
+  // waves stride over the packed rows.
+  const std::size_t rows_per_block = kJThreads / 64;
+  const unsigned blocks = static_cast<unsigned>(std::min<std::size_t>((npair + rows_per_block - 1) / rows_per_block, 512));
   if (prec == Precision::FP64)
-    hipLaunchKernelGGL((coulomb_matrix_kernel<double, 2>), dim3(blocks), dim3(256), lds, s,
+    hipLaunchKernelGGL(coulomb_matrix_kernel<double>, dim3(blocks), dim3(kJThreads), lds, s,
                        static_cast<const double*>(B), rho, npair, ldq, n, J);
   else
-    hipLaunchKernelGGL((coulomb_matrix_kernel<float, 4>), dim3(blocks), dim3(256), lds, s,
+    hipLaunchKernelGGL(coulomb_matrix_kernel<float>, dim3(blocks), dim3(kJThreads), lds, s,
                        static_cast<const float*>(B), rho, npair, ldq, n, J);
   VQC_LAUNCH("coulomb_matrix_kernel");
 }
@@ -665,35 +823,53 @@ void prepare_occupied(Precision prec, OccupiedOperands& ops, hipStream_t s) {
   }
 }
 
-void occupied_u(Precision prec, const void* B, const OccupiedOperands& ops, int ldq, void* U, hipStream_t s) {
+void occupied_u(Precision prec, const void* B, const OccupiedOperands& ops, int ldq, void* U, const double* D,
+                double* rho_part, hipStream_t s) {
   switch (ops.ni) {
-    case 1: launch_u<1>(prec, B, ops, ldq, U, s); break;
-    case 2: launch_u<2>(prec, B, ops, ldq, U, s); break;
-    case 3: launch_u<3>(prec, B, ops, ldq, U, s); break;
-    case 4: launch_u<4>(prec, B, ops, ldq, U, s); break;
-    case 5: launch_u<5>(prec, B, ops, ldq, U, s); break;
-    case 6: launch_u<6>(prec, B, ops, ldq, U, s); break;
-    case 7: launch_u<7>(prec, B, ops, ldq, U, s); break;
-    case 8: launch_u<8>(prec, B, ops, ldq, U, s); break;
-    case 9: launch_u<9>(prec, B, ops, ldq, U, s); break;
-    case 10: launch_u<10>(prec, B, ops, ldq, U, s); break;
+    case 1: launch_u<1>(prec, B, ops, ldq, U, D, rho_part, s); break;
+    case 2: launch_u<2>(prec, B, ops, ldq, U, D, rho_part, s); break;
+    case 3: launch_u<3>(prec, B, ops, ldq, U, D, rho_part, s); break;
+    case 4: launch_u<4>(prec, B, ops, ldq, U, D, rho_part, s); break;
+    case 5: launch_u<5>(prec, B, ops, ldq, U, D, rho_part, s); break;
+    case 6: launch_u<6>(prec, B, ops, ldq, U, D, rho_part, s); break;
+    case 7: launch_u<7>(prec, B, ops, ldq, U, D, rho_part, s); break;
+    case 8: launch_u<8>(prec, B, ops, ldq, U, D, rho_part, s); break;
+    case 9: launch_u<9>(prec, B, ops, ldq, U, D, rho_part, s); break;
+    case 10: launch_u<10>(prec, B, ops, ldq, U, D, rho_part, s); break;
     default: throw std::logic_error("occupied tile width out of range");
   }
 }
 
 namespace {
-// Split count of the large-tile BF16X3 SYRK: about 4 blocks per CU, with the
-// partial slices of one run capped at 256 MiB.
+// Split-K slices per launch: at most 256 MB of FP64 partial slices (and 128).
 int syrk_split_cap(int n) {
   const std::size_t slice = static_cast<std::size_t>(n) * n * sizeof(double);
-  return static_cast<int>(std::max<std::size_t>(1, std::min<std::size_t>(256, (256ull << 20) / slice)));
+  return static_cast<int>(std::max<std::size_t>(1, std::min<std::size_t>(128, (256ull << 20) / slice)));
 }
-int syrk_tiles(int n, int& ncb) {
-  ncb = (n + kSyrkTN - 1) / kSyrkTN;
-  const int nrb = (n + kSyrkTM - 1) / kSyrkTM;
+int syrk_tiles(int n, int tm, int tn, int& ncb) {
+  ncb = (n + tn - 1) / tn;
+  const int nrb = (n + tm - 1) / tm;
   int t = 0;
-  for (int rb = 0; rb < nrb; ++rb) t += std::min(2 * (rb + 1), ncb);
+  for (int rb = 0; rb < nrb; ++rb) t += std::min((tm / tn) * (rb + 1), ncb);
   return t;
+}
+// Equal-size work items fill the GPU in whole waves of `slots` resident
+// blocks: take the fewest splits whose tiles x splits fill at least 95% of the
+// last wave, else the best fill under the cap.
+int syrk_splits(int tiles, int cap, int slots) {
+  const int kSlots = slots;
+  int best = 1;
+  double best_fill = 0.0;
+  for (int sp = 1; sp <= cap; ++sp) {
+    const int w = tiles * sp;
+    const double fill = static_cast<double>(w) / (static_cast<double>((w + kSlots - 1) / kSlots) * kSlots);
+    if (w >= kSlots && fill >= 0.95) return sp;
+    if (fill > best_fill + 1e-12) {
+      best = sp;
+      best_fill = fill;
+    }
+  }
+  return best;
 }
 }  // namespace
 
@@ -710,9 +886,13 @@ void exchange_syrk(Precision prec, const void* U, int n, int r, int r_pos, int l
     const std::size_t len = ke - kb;
     double* out = partial + static_cast<std::size_t>(slices) * n * n;
     int ncb = 0;
-    const int tiles = syrk_tiles(n, ncb);
-    const std::size_t bk = prec == Precision::BF16X3 ? 32 : 16;
-    int splits = std::max(1, std::min(syrk_split_cap(n), (1024 + tiles - 1) / tiles));
+    const bool bf = prec == Precision::BF16X3;
+    const int tiles = bf ? syrk_tiles(n, kSyrkBfT, kSyrkBfT, ncb) : syrk_tiles(n, kSyrkTM, kSyrkTN, ncb);
+    const std::size_t bk = bf ? 32 : 16;
+    // Resident 512-thread blocks on 256 CUs: one per CU for BF16X3 (80 KB LDS)
+    // and FP64 (102 KB), two for FP32 (51 KB).
+    const int slots = prec == Precision::FP32 ? 512 : 256;
+    int splits = syrk_splits(tiles, syrk_split_cap(n), slots);
     const std::size_t chunk = round_up((len + splits - 1) / splits, bk);
     splits = static_cast<int>((len + chunk - 1) / chunk);
     const int nwork = tiles * splits;

@@ -30,6 +30,8 @@ void pack_density(const double* D, int n, double* dpacked, hipStream_t s);
 std::size_t coulomb_partial_size(std::size_t npair, int ldq);
 void coulomb_rho(Precision prec, const void* B, const double* dpacked, std::size_t npair, int ldq, double* rho,
                  double* partial, hipStream_t s);
+// rho = sum of `rows` partial rows (as written by occupied_u).
+void rho_from_partials(const double* rho_part, int rows, int ldq, double* rho, hipStream_t s);
 void coulomb_matrix(Precision prec, const void* B, const double* rho, std::size_t npair, int ldq, int n, double* J,
                     hipStream_t s);
 
@@ -43,8 +45,11 @@ struct OccupiedOperands {
   void* ct_lo = nullptr;      // lo part
 };
 void prepare_occupied(Precision prec, OccupiedOperands& ops, hipStream_t s);
-// U[mu][i][Q] (r x ldq per mu) in FP64 (fp64) or FP32 (fp32/bf16x3).
-void occupied_u(Precision prec, const void* B, const OccupiedOperands& ops, int ldq, void* U, hipStream_t s);
+// U[mu][i][Q] (r x ldq per mu): FP64 (fp64), FP32 (fp32), or hi/lo bfloat16
+// planes (bf16x3).  With rho_part non-null the same pass writes
+// rho_part[mu][Q] = sum_nu D[mu][nu] B[(mu,nu)][Q] (n x ldq, FP64).
+void occupied_u(Precision prec, const void* B, const OccupiedOperands& ops, int ldq, void* U, const double* D,
+                double* rho_part, hipStream_t s);
 // K (n x n, FP64, both triangles) = sum_k sign_k W[mu][k] W[lam][k] where
 // columns i < r_pos carry +1 and the rest -1.  `partial` needs
 // exchange_partial_size(n) doubles.

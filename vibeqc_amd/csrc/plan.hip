@@ -260,7 +260,7 @@ void DfPlan::ensure_factor(int r) {
     D_.resize(nn);
     dpacked_.resize(npair_);
     rho_.resize(ldq_);
-    rho_partial_.resize(coulomb_partial_size(npair_, ldq_));
+    rho_partial_.resize(std::max(coulomb_partial_size(npair_, ldq_), static_cast<std::size_t>(n_) * ldq_));
     syrk_partial_.resize(exchange_partial_size(n_));
   }
   r = std::max(r, 1);
@@ -291,17 +291,24 @@ void DfPlan::ensure_factor(int r) {
 void DfPlan::build_jk(Precision p, const double* D, int r, int r_pos) {
   if (p != Precision::FP64) ensure_reduced();
   const void* B = p == Precision::FP64 ? static_cast<const void*>(B64_.data()) : static_cast<const void*>(B32_.data());
+  // When the U pass runs it also produces the rho partials, so J needs only
+  // one more pass over B.
+  const bool fused_rho = opts_.exchange && opts_.coulomb && r > 0;
   if (opts_.exchange) {
     ops_.r = r;
     if (r > 0) {
       prepare_occupied(p, ops_, stream_);
-      occupied_u(p, B, ops_, ldq_, U_.data(), stream_);
+      occupied_u(p, B, ops_, ldq_, U_.data(), D, fused_rho ? rho_partial_.data() : nullptr, stream_);
     }
     exchange_syrk(p, U_.data(), n_, r, r_pos, ldq_, K_.data(), syrk_partial_.data(), stream_);
   }
   if (opts_.coulomb) {
-    pack_density(D, n_, dpacked_.data(), stream_);
-    coulomb_rho(p, B, dpacked_.data(), npair_, ldq_, rho_.data(), rho_partial_.data(), stream_);
+    if (fused_rho) {
+      rho_from_partials(rho_partial_.data(), n_, ldq_, rho_.data(), stream_);
+    } else {
+      pack_density(D, n_, dpacked_.data(), stream_);
+      coulomb_rho(p, B, dpacked_.data(), npair_, ldq_, rho_.data(), rho_partial_.data(), stream_);
+    }
     coulomb_matrix(p, B, rho_.data(), npair_, ldq_, n_, J_.data(), stream_);
   }
 }
@@ -424,12 +431,11 @@ std::array<double, 5> DfPlan::benchmark_occupied(const std::vector<double>& occu
     VQC_HIP(hipEventRecord(ev[it * 4 + 0].e, stream_));
     ops_.r = nocc;
     prepare_occupied(p, ops_, stream_);
-    occupied_u(p, B, ops_, ldq_, U_.data(), stream_);
+    occupied_u(p, B, ops_, ldq_, U_.data(), D_.data(), rho_partial_.data(), stream_);
     VQC_HIP(hipEventRecord(ev[it * 4 + 1].e, stream_));
     exchange_syrk(p, U_.data(), n_, nocc, nocc, ldq_, K_.data(), syrk_partial_.data(), stream_);
     VQC_HIP(hipEventRecord(ev[it * 4 + 2].e, stream_));
-    pack_density(D_.data(), n_, dpacked_.data(), stream_);
-    coulomb_rho(p, B, dpacked_.data(), npair_, ldq_, rho_.data(), rho_partial_.data(), stream_);
+    rho_from_partials(rho_partial_.data(), n_, ldq_, rho_.data(), stream_);
     coulomb_matrix(p, B, rho_.data(), npair_, ldq_, n_, J_.data(), stream_);
     VQC_HIP(hipEventRecord(ev[it * 4 + 3].e, stream_));
   }
