@@ -22,7 +22,10 @@ Pinning (each function is checked by tests/test_oracle_*.py against):
     the fixture density;
   * rhf_scf: the PySCF DF-RHF energies in posthf/*.json and the GPU4PySCF
     cc-pVDZ/cc-pVDZ-JKFIT energies in
-    benchmarks/results/issue409-packed-values/references/*-reference.json.
+    benchmarks/results/issue409-packed-values/references/*-reference.json;
+  * df_rhf_gradient: central finite differences of df_rhf energies, the
+    translational invariance sum_A dE/dR_A = 0, and the GPU4PySCF forces in
+    the same issue409 reference JSONs.
 """
 
 from __future__ import annotations
@@ -67,6 +70,10 @@ def library():
         lib.oracle_one_electron.argtypes = [b, ctypes.c_int, d, d, d, d, d]
         lib.oracle_boys.argtypes = [ctypes.c_int, ctypes.c_double, d]
         lib.oracle_boys.restype = None
+        i = ctypes.POINTER(ctypes.c_int)
+        lib.oracle_grad_three_center.argtypes = [b, b, i, i, d, ctypes.c_int, d]
+        lib.oracle_grad_metric.argtypes = [b, i, d, ctypes.c_int, d]
+        lib.oracle_grad_one_electron.argtypes = [b, ctypes.c_int, d, d, i, d, d, d]
         _lib = lib
     return _lib
 
@@ -533,3 +540,108 @@ def build_problem(atoms_z, atoms_xyz, shells, spherical, aux_shells, aux_spheric
 
 def df_rhf(problem: DfProblem, **kw) -> ScfResult:
     return rhf_scf(problem.S, problem.H, problem.enuc, problem.nocc, lambda D: jk(problem.B, D), **kw)
+
+
+# ----------------------------------------------------------------------------
+# Nuclear gradient of the DF-RHF energy (density_fitting.cpp:898-917, 464-595;
+# density_fitting_metric.cpp:50-99; hf_gradient.cpp:8-18; mean_field.cpp:18-24)
+# ----------------------------------------------------------------------------
+
+
+def _atom_of(shells):
+    return np.array([s.atom_index for s in shells], dtype=np.int32)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def grad_three_center(atoms_xyz, natom, shells, spherical, aux_shells, aux_spherical, w) -> np.ndarray:
+    """sum_{mu nu P} w[mu, nu, P] d(mu nu|P)/dR, (natom, 3); w in public AOs."""
+    xyz = np.ascontiguousarray(atoms_xyz, dtype=np.float64)
+    po, pa = _Packed(xyz, shells), _Packed(xyz, aux_shells)
+    U = transform_matrix(shells, spherical)
+    Wx = transform_matrix(aux_shells, aux_spherical)
+    wc = np.einsum("mnp,ma->anp", w, U, optimize=True)
+    wc = np.einsum("anp,nb->abp", wc, U, optimize=True)
+    wc = np.ascontiguousarray(np.einsum("abp,pk->abk", wc, Wx, optimize=True))
+    g = np.zeros((natom, 3))
+    ao, aa = _atom_of(shells), _atom_of(aux_shells)
+    if library().oracle_grad_three_center(ctypes.byref(po.c), ctypes.byref(pa.c), _iptr(ao), _iptr(aa), _dptr(wc), natom, _dptr(g)):
+        raise ValueError("angular momentum above the oracle limit")
+    return g
+
+
+def grad_metric(atoms_xyz, natom, aux_shells, aux_spherical, w) -> np.ndarray:
+    """sum_{PQ} w[P, Q] d(P|Q)/dR, (natom, 3); w in public auxiliary order."""
+    xyz = np.ascontiguousarray(atoms_xyz, dtype=np.float64)
+    pa = _Packed(xyz, aux_shells)
+    Wx = transform_matrix(aux_shells, aux_spherical)
+    wc = np.ascontiguousarray(Wx.T @ w @ Wx)
+    g = np.zeros((natom, 3))
+    aa = _atom_of(aux_shells)
+    if library().oracle_grad_metric(ctypes.byref(pa.c), _iptr(aa), _dptr(wc), natom, _dptr(g)):
+        raise ValueError("angular momentum above the oracle limit")
+    return g
+
+
+def grad_one_electron(atoms_z, atoms_xyz, shells, spherical, D, W) -> np.ndarray:
+    """sum D d(T + V)/dR - sum W dS/dR, (natom, 3) (hf_gradient.cpp:8-18)."""
+    xyz = np.ascontiguousarray(atoms_xyz, dtype=np.float64)
+    z = np.ascontiguousarray(atoms_z, dtype=np.float64)
+    pk = _Packed(xyz, shells)
+    U = transform_matrix(shells, spherical)
+    dc = np.ascontiguousarray(U.T @ D @ U)
+    wc = np.ascontiguousarray(U.T @ W @ U)
+    g = np.zeros((len(z), 3))
+    ao = _atom_of(shells)
+    if library().oracle_grad_one_electron(ctypes.byref(pk.c), len(z), _dptr(z), _dptr(xyz), _iptr(ao), _dptr(dc), _dptr(wc), _dptr(g)):
+        raise ValueError("angular momentum above the oracle limit")
+    return g
+
+
+def nuclear_repulsion_gradient(atoms_z, atoms_xyz) -> np.ndarray:
+    z = np.asarray(atoms_z, dtype=np.float64)
+    x = np.asarray(atoms_xyz, dtype=np.float64)
+    d = x[:, None, :] - x[None, :, :]
+    r = np.linalg.norm(d, axis=-1)
+    np.fill_diagonal(r, np.inf)
+    return -np.einsum("i,j,ijx->ix", z, z, d / r[..., None] ** 3)
+
+
+def df_weights(A, M, D, C_occ):
+    """Response weights of the DF two-electron energy for a full-rank metric
+    (density_fitting.cpp:464-595): dE2 = sum W_A dA + sum W_M dM with
+      c = M^-1 rho, rho_P = sum D A_P,   T_Pij = sum C_mi C_nj A_mnP,
+      Y = M^-1 T,
+      W_A = D c_P - 2 sum_ij C_mi C_nj Y_Pij,
+      W_M = -1/2 c c^T + sum_ij Y_Pij Y_Qij
+    (E_J = 1/2 rho^T M^-1 rho and E_K = -sum_ij T_ij^T M^-1 T_ij for
+    D = 2 C_occ C_occ^T)."""
+    Minv = np.linalg.inv(M)
+    rho = np.einsum("mn,mnp->p", D, A, optimize=True)
+    c = Minv @ rho
+    T = np.einsum("mi,nj,mnp->pij", C_occ, C_occ, A, optimize=True)
+    Y = np.einsum("pq,qij->pij", Minv, T, optimize=True)
+    WA = np.einsum("mn,p->mnp", D, c) - 2.0 * np.einsum("mi,nj,pij->mnp", C_occ, C_occ, Y, optimize=True)
+    WM = -0.5 * np.outer(c, c) + np.einsum("pij,qij->pq", Y, Y, optimize=True)
+    return WA, WM
+
+
+def df_rhf_gradient(atoms_z, atoms_xyz, shells, spherical, aux_shells, aux_spherical, problem: DfProblem, scf: ScfResult):
+    """dE/dR (natom, 3) of a converged DF-RHF energy; forces are its negative
+    (hf_gradient.cpp:8-18 with the DF two-electron response of
+    density_fitting.cpp:898-917)."""
+    if problem.rank != problem.M.shape[0]:
+        raise ValueError("the gradient oracle assumes a full-rank metric")
+    natom = len(atoms_z)
+    Co = scf.coefficients[:, : problem.nocc]
+    eps = scf.orbital_energies[: problem.nocc]
+    D = 2.0 * Co @ Co.T
+    W = 2.0 * (Co * eps) @ Co.T
+    WA, WM = df_weights(problem.A, problem.M, D, Co)
+    g = nuclear_repulsion_gradient(atoms_z, atoms_xyz)
+    g += grad_one_electron(atoms_z, atoms_xyz, shells, spherical, D, W)
+    g += grad_three_center(atoms_xyz, natom, shells, spherical, aux_shells, aux_spherical, WA)
+    g += grad_metric(atoms_xyz, natom, aux_shells, aux_spherical, WM)
+    return g

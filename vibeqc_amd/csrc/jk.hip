@@ -108,12 +108,27 @@ __global__ __launch_bounds__(kRhoThreads) void coulomb_rho_kernel(const T* __res
   for (int v = 0; v < V; ++v) out[v] = acc[v];
 }
 
-__global__ void sum_rows_kernel(const double* __restrict__ partial, int rows, int cols, double* __restrict__ out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
+// out[c] = sum_r partial[r][c]: 64 columns per block, 16 row groups per
+// column reduced through LDS in a fixed order (deterministic).
+constexpr int kSumCols = 64, kSumGroups = 16;
+__global__ __launch_bounds__(kSumCols * kSumGroups) void sum_rows_kernel(const double* __restrict__ partial, int rows,
+                                                                        int cols, double* __restrict__ out) {
+  __shared__ double red[kSumGroups][kSumCols];
+  const int lane = threadIdx.x % kSumCols, g = threadIdx.x / kSumCols;
+  const int c = blockIdx.x * kSumCols + lane;
   double s = 0.0;
-  for (int r = 0; r < rows; ++r) s += partial[static_cast<std::size_t>(r) * cols + c];
-  out[c] = s;
+  if (c < cols) {
+#pragma unroll 4
+    for (int r = g; r < rows; r += kSumGroups) s += partial[static_cast<std::size_t>(r) * cols + c];
+  }
+  red[g][lane] = s;
+  __syncthreads();
+  if (g == 0 && c < cols) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < kSumGroups; ++k) t += red[k][lane];
+    out[c] = t;
+  }
 }
 
 template <class T>
@@ -309,7 +324,7 @@ __global__ __launch_bounds__(128 * WQ) void occupied_u_kernel(const T* __restric
 // chunk per plane.  C^T arrives pre-split (ldc rows of ldct K-contiguous
 // entries, zero beyond n).  U is written as hi/lo planes.  The next k-step is
 // staged in registers while the current one runs on the MFMAs.
-template <int NIW>
+template <int NIW, int WQ>
 __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __restrict__ B,
                                                                 const __bf16* __restrict__ Cth,
                                                                 const __bf16* __restrict__ Ctl,
@@ -317,8 +332,12 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
                                                                 int n, int ldq, int ldct, int r, int nqb,
                                                                 const double* __restrict__ D,
                                                                 double* __restrict__ rho_part) {
-  // 16 waves: 8 along Q (32 rows each) x 2 along i (16*NIW columns each).
+  // 16 waves: WQ along Q x (16 / WQ) along i.  WQ = 4 gives 64 x 8*NIW wave
+  // tiles (even NIW), which read fewer LDS fragments per MFMA than WQ = 8
+  // (32 x 16*NIW).
   constexpr int BQ = kUTileQ, BK = 32, BI = 32 * NIW, PAD = 8, NT = 1024;
+  constexpr int WI = 16 / WQ, XF = BQ / (16 * WQ), YF = BI / (16 * WI);
+  static_assert(YF * 16 * WI == BI, "i tile must split evenly over the waves");
   constexpr int YP = (BI * 4 + NT - 1) / NT;  // 16-byte C chunks per thread and plane
   __shared__ __bf16 Xh[BQ][BK + PAD], Xl[BQ][BK + PAD];
   __shared__ __bf16 Yh[BI][BK + PAD], Yl[BI][BK + PAD];
@@ -326,12 +345,12 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
   const int q0 = (blockIdx.x % nqb) * BQ;
   const int i0 = blockIdx.y * BI;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wq = wave & 7, wi = wave >> 3;
-  f4 acc[2][NIW];
+  const int wq = wave % WQ, wi = wave / WQ;
+  f4 acc[XF][YF];
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int x = 0; x < XF; ++x)
 #pragma unroll
-    for (int y = 0; y < NIW; ++y) acc[x][y] = f4{};
+    for (int y = 0; y < YF; ++y) acc[x][y] = f4{};
   const int xq = tid % BQ, xg = tid / BQ;  // one Q column, 8 consecutive nu
   float rx[8];
   const bool with_rho = rho_part != nullptr && blockIdx.y == 0;
@@ -389,22 +408,24 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
     const bool more = nu0 + BK < n;
     if (more) gload(nu0 + BK);
     const int kq = 8 * (lane >> 4);
-    const bf8 ah0 = *reinterpret_cast<const bf8*>(&Xh[wq * 32 + (lane & 15)][kq]);
-    const bf8 al0 = *reinterpret_cast<const bf8*>(&Xl[wq * 32 + (lane & 15)][kq]);
-    const bf8 ah1 = *reinterpret_cast<const bf8*>(&Xh[wq * 32 + 16 + (lane & 15)][kq]);
-    const bf8 al1 = *reinterpret_cast<const bf8*>(&Xl[wq * 32 + 16 + (lane & 15)][kq]);
+    bf8 ah[XF], al[XF];
 #pragma unroll
-    for (int y = 0; y < NIW; ++y) {
-      const int row = wi * 16 * NIW + y * 16 + (lane & 15);
+    for (int x = 0; x < XF; ++x) {
+      ah[x] = *reinterpret_cast<const bf8*>(&Xh[wq * 16 * XF + x * 16 + (lane & 15)][kq]);
+      al[x] = *reinterpret_cast<const bf8*>(&Xl[wq * 16 * XF + x * 16 + (lane & 15)][kq]);
+    }
+#pragma unroll
+    for (int y = 0; y < YF; ++y) {
+      const int row = wi * 16 * YF + y * 16 + (lane & 15);
       const bf8 bh = *reinterpret_cast<const bf8*>(&Yh[row][kq]);
       const bf8 bl = *reinterpret_cast<const bf8*>(&Yl[row][kq]);
-      // Alternate the two accumulators so no MFMA waits on its predecessor.
-      acc[0][y] = mfma(ah0, bh, acc[0][y]);
-      acc[1][y] = mfma(ah1, bh, acc[1][y]);
-      acc[0][y] = mfma(ah0, bl, acc[0][y]);
-      acc[1][y] = mfma(ah1, bl, acc[1][y]);
-      acc[0][y] = mfma(al0, bh, acc[0][y]);
-      acc[1][y] = mfma(al1, bh, acc[1][y]);
+      // XF independent accumulators between dependent MFMAs keep the pipe full.
+#pragma unroll
+      for (int x = 0; x < XF; ++x) acc[x][y] = mfma(ah[x], bh, acc[x][y]);
+#pragma unroll
+      for (int x = 0; x < XF; ++x) acc[x][y] = mfma(ah[x], bl, acc[x][y]);
+#pragma unroll
+      for (int x = 0; x < XF; ++x) acc[x][y] = mfma(al[x], bh, acc[x][y]);
     }
     __syncthreads();
     if (more) {
@@ -424,11 +445,11 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
     }
   }
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int x = 0; x < XF; ++x)
 #pragma unroll
-    for (int y = 0; y < NIW; ++y) {
-      const int i = i0 + wi * 16 * NIW + y * 16 + (lane & 15);
-      const int qb = q0 + wq * 32 + x * 16;
+    for (int y = 0; y < YF; ++y) {
+      const int i = i0 + wi * 16 * YF + y * 16 + (lane & 15);
+      const int qb = q0 + wq * 16 * XF + x * 16;
       if (i >= r || qb >= ldq) continue;
       const std::size_t off = (static_cast<std::size_t>(mu) * r + i) * ldq + qb + 4 * (lane >> 4);
 #pragma unroll
@@ -732,7 +753,9 @@ void launch_u(Precision prec, const void* B, const OccupiedOperands& ops, int ld
   } else {
     const int nqb = (ldq + kUTileQ - 1) / kUTileQ;
     dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (32 * NIW));
-    hipLaunchKernelGGL((occupied_u_bf16x3_kernel<NIW>), grid, dim3(1024), dyn, s, static_cast<const float*>(B),
+    // 8 x 2 waves: the 4 x 4 layout needs more than the 128 registers a
+    // 16-wave block allows once NIW >= 8.
+    hipLaunchKernelGGL((occupied_u_bf16x3_kernel<NIW, 8>), grid, dim3(1024), dyn, s, static_cast<const float*>(B),
                        static_cast<const __bf16*>(ops.ct_hi), static_cast<const __bf16*>(ops.ct_lo),
                        static_cast<__bf16*>(U), static_cast<__bf16*>(U) + static_cast<std::size_t>(ops.n) * ops.r * ldq,
                        ops.n, ldq, ops.ldct, ops.r, nqb, D, rho_part);
@@ -783,12 +806,14 @@ void coulomb_rho(Precision prec, const void* B, const double* dpacked, std::size
     hipLaunchKernelGGL((coulomb_rho_kernel<float, 4>), grid, dim3(kRhoThreads), 0, s,
                        static_cast<const float*>(B), dpacked, npair, ldq, chunk, partial);
   VQC_LAUNCH("coulomb_rho_kernel");
-  hipLaunchKernelGGL(sum_rows_kernel, dim3((ldq + 255) / 256), dim3(256), 0, s, partial, chunks, ldq, rho);
+  hipLaunchKernelGGL(sum_rows_kernel, dim3((ldq + kSumCols - 1) / kSumCols), dim3(kSumCols * kSumGroups), 0, s, partial,
+                     chunks, ldq, rho);
   VQC_LAUNCH("sum_rows_kernel");
 }
 
 void rho_from_partials(const double* rho_part, int rows, int ldq, double* rho, hipStream_t s) {
-  hipLaunchKernelGGL(sum_rows_kernel, dim3((ldq + 255) / 256), dim3(256), 0, s, rho_part, rows, ldq, rho);
+  hipLaunchKernelGGL(sum_rows_kernel, dim3((ldq + kSumCols - 1) / kSumCols), dim3(kSumCols * kSumGroups), 0, s,
+                     rho_part, rows, ldq, rho);
   VQC_LAUNCH("sum_rows_kernel");
 }
 
