@@ -628,6 +628,30 @@ def df_weights(A, M, D, C_occ):
     return WA, WM
 
 
+def df_weights_dense(A, M, D, cj=1.0, ck=-0.5):
+    """Response weights of E2 = 1/2 sum D (cJ J + cK K) at a fixed symmetric
+    (possibly indefinite) density, without a factor of D:
+      At_P = sum_Q (M^-1)_PQ A_Q,   c = M^-1 sum D A,
+      W_A[P] = cJ D c_P + cK D At_P D,
+      W_M[P, Q] = -cJ/2 c_P c_Q - cK/2 tr(D At_P D At_Q)."""
+    Minv = np.linalg.inv(M)
+    At = np.einsum("mnq,pq->mnp", A, Minv, optimize=True)
+    c = np.einsum("mn,mnp->p", D, At, optimize=True)
+    DAD = np.einsum("ml,lsp,sn->mnp", D, At, D, optimize=True)
+    WA = cj * np.einsum("mn,p->mnp", D, c) + ck * DAD
+    WM = -0.5 * cj * np.outer(c, c) - 0.5 * ck * np.einsum("mnp,mnq->pq", DAD, At, optimize=True)
+    return WA, WM
+
+
+def df_two_electron_gradient(atoms_xyz, natom, shells, spherical, aux_shells, aux_spherical, A, M, D, cj=1.0, ck=-0.5):
+    """Fixed-density dE2/dR (natom, 3) of 1/2 sum D (cJ J + cK K): the
+    gradient the plan's evaluate(derivative=True) returns (fock.h:73-80)."""
+    WA, WM = df_weights_dense(A, M, D, cj, ck)
+    return grad_three_center(atoms_xyz, natom, shells, spherical, aux_shells, aux_spherical, WA) + grad_metric(
+        atoms_xyz, natom, aux_shells, aux_spherical, WM
+    )
+
+
 def df_rhf_gradient(atoms_z, atoms_xyz, shells, spherical, aux_shells, aux_spherical, problem: DfProblem, scf: ScfResult):
     """dE/dR (natom, 3) of a converged DF-RHF energy; forces are its negative
     (hf_gradient.cpp:8-18 with the DF two-electron response of

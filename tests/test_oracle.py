@@ -227,3 +227,45 @@ def test_gradient_planted_mistake_is_caught(oracle):
     broken = g - oracle.grad_metric(case.xyz, len(case.atoms), ax, case.spherical, WM)
     forces = np.array(meta["forces_hartree_per_bohr"])
     assert np.abs(-broken - forces).max() > 1e-4
+
+
+def test_dense_weights_equal_occupied_weights(oracle):
+    """For D = 2 C C^T the factor-free weights equal the occupied-factor ones."""
+    case, _ = ccpvdz_case("ccpvdz_water")
+    _, problem, scf = _oracle_gradient(oracle, case)
+    Co = scf.coefficients[:, : problem.nocc]
+    D = 2.0 * Co @ Co.T
+    wa1, wm1 = oracle.df_weights(problem.A, problem.M, D, Co)
+    wa2, wm2 = oracle.df_weights_dense(problem.A, problem.M, D)
+    # Different contraction orders through M^-1 agree to roundoff.
+    assert np.abs(wa1 - wa2).max() < 1e-10 * np.abs(wa1).max()
+    assert np.abs(wm1 - wm2).max() < 1e-10 * np.abs(wm1).max()
+
+
+def test_fixed_density_two_electron_gradient_central_differences(oracle):
+    """dE2/dR at a fixed indefinite AO density (the basis moves with the atoms)
+    against central differences of 1/2 sum D (J - K/2)."""
+    case, _, _ = posthf_case("lih")
+    sh, ax = case.oracle_shells(oracle), case.oracle_shells(oracle, True)
+    n = sum(2 * s.l + 1 if case.spherical else (s.l + 1) * (s.l + 2) // 2 for s in sh)
+    rng = np.random.default_rng(11)
+    D = rng.normal(size=(n, n))
+    D = D + D.T
+
+    def e2(xyz):
+        A = oracle.three_center(xyz, sh, case.spherical, ax, case.spherical)
+        M = oracle.metric(xyz, ax, case.spherical)
+        X, _, _ = oracle.metric_inverse_sqrt(M)
+        J, K = oracle.jk(oracle.df_tensor(A, X), D)
+        return oracle.two_electron_energy(D, J, K), A, M
+
+    _, A, M = e2(case.xyz)
+    g = oracle.df_two_electron_gradient(case.xyz, len(case.atoms), sh, case.spherical, ax, case.spherical, A, M, D)
+    h = 1e-4
+    for a in range(len(case.atoms)):
+        for x in range(3):
+            xp, xm = case.xyz.copy(), case.xyz.copy()
+            xp[a, x] += h
+            xm[a, x] -= h
+            fd = (e2(xp)[0] - e2(xm)[0]) / (2 * h)
+            assert abs(g[a, x] - fd) < 1e-7 * max(1.0, abs(fd)), (a, x, g[a, x], fd)

@@ -22,8 +22,14 @@ CSRC = PACKAGE / "csrc"
 BUILD = PACKAGE / "_build"
 LIBRARY = PACKAGE / "libvibeqc_amd.so"
 CLASS_UNITS = tuple(f"integrals_c{a}{b}.hip" for a in range(3) for b in range(a + 1)) + ("integrals_c30.hip",)
+GRAD_UNITS = tuple(f"integrals_g{a}{b}.hip" for a in range(3) for b in range(a + 1)) + ("integrals_g30.hip",)
 # Largest translation units first so the parallel build finishes soonest.
-SOURCES = tuple(reversed(CLASS_UNITS)) + ("integrals.hip", "jk.hip", "plan.hip", "scf.hip", "basis.cpp", "capi.cpp")
+SOURCES = (
+    ("integrals_g22.hip", "integrals_g21.hip")
+    + tuple(reversed(CLASS_UNITS))
+    + tuple(u for u in reversed(GRAD_UNITS) if u not in ("integrals_g22.hip", "integrals_g21.hip"))
+    + ("integrals.hip", "integrals_grad.hip", "jk.hip", "plan.hip", "gradient.hip", "scf.hip", "basis.cpp", "capi.cpp")
+)
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 
 COMMON_FLAGS = [

@@ -128,13 +128,24 @@ class Calculator:
             precision=_PRECISIONS[self.precision],
         )
 
-    def singlepoint(self, atoms, *, charge: int = 0, multiplicity: int = 1, properties=("energy",)) -> Result:
-        """Energy (Hartree) of one system; forces are not yet provided."""
-        props = tuple(properties) if properties is not None else ("energy",)
-        if "forces" in props:
-            raise NotImplementedError("analytic forces are not yet provided by the HIP engine")
+    def singlepoint(self, atoms, *, charge: int = 0, multiplicity: int = 1, properties=None) -> Result:
+        """Energy (Hartree) and, unless ``properties=("energy",)``, analytic
+        forces -dE/dR (natom, 3) in Hartree/bohr (calculator.py:2128-2200:
+        the default requests every supported property)."""
+        if properties is None:
+            requested = frozenset({"energy", "forces"})
+        else:
+            if isinstance(properties, (str, bytes)):
+                raise TypeError("properties must be an iterable of property names")
+            requested = frozenset(properties)
+        if "energy" not in requested:
+            raise ValueError("properties must include 'energy'")
+        unknown = requested - {"energy", "forces"}
+        if unknown:
+            raise ValueError(f"method 'rhf' does not support properties: {', '.join(sorted(map(repr, unknown)))}")
         with self.prepare(atoms, charge=charge, multiplicity=multiplicity) as plan:
             r = plan.solve(
+                compute_forces="forces" in requested,
                 max_iterations=self.max_iterations,
                 diis_history=self.diis_history,
                 energy_tolerance=self.energy_tolerance,
@@ -143,7 +154,7 @@ class Calculator:
             shape = plan.shape()
         return Result(
             r.energy,
-            None,
+            r.forces,
             r.iterations,
             True,
             r.energy_change,

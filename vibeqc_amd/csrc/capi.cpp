@@ -211,8 +211,8 @@ generativeqc_status generativeqc_fock_plan_create(generativeqc_context* context,
             "Fock metric cutoff must be finite and in [0, 1)");
     if (spec->spin != GENERATIVEQC_FOCK_RESTRICTED)
       throw vqc::NotImplemented("only restricted (closed-shell) Fock plans are implemented");
-    if (spec->derivative_order != 0)
-      throw vqc::NotImplemented("analytic derivatives are not yet provided by this engine");
+    if (spec->derivative_order > 1)
+      throw vqc::NotImplemented("only first nuclear derivatives are implemented");
     if (!term_ok(spec->coulomb) || !term_ok(spec->exchange))
       throw vqc::NotImplemented("only full-range density-fitted J/K terms are implemented");
     if (!spec->coulomb.present && !spec->exchange.present)
@@ -256,16 +256,21 @@ generativeqc_status generativeqc_fock_plan_evaluate(generativeqc_fock_plan* plan
     const uint64_t matrix = static_cast<uint64_t>(p.nbf()) * p.nbf();
     require(density_count == matrix && result->matrix_count == matrix && beta == nullptr && beta_count == 0,
             "Fock density/output shape or spin mismatch");
-    require(result->gradient ? false : result->gradient_count == 0, "Fock gradient capability/count mismatch");
+    const uint64_t coordinates = static_cast<uint64_t>(p.system().atoms.size()) * 3;
+    require(result->gradient ? plan->resolved.derivative_order == 1 && result->gradient_count == coordinates
+                             : result->gradient_count == 0,
+            "Fock gradient capability/count mismatch");
     require(result->exchange_beta == nullptr && result->fock_beta == nullptr,
             "restricted plans have no beta outputs");
     const auto bytes = buffer_bytes(matrix);
     disjoint_outputs({{result->coulomb, bytes},
                       {result->exchange_alpha, bytes},
                       {result->fock_alpha, bytes},
+                      {result->gradient, buffer_bytes(result->gradient_count)},
                       {result, sizeof(*result)}});
     const std::vector<double> d(density, density + matrix);
-    const vqc::JkResult jk = p.evaluate(d);
+    std::vector<double> gradient;
+    const vqc::JkResult jk = p.evaluate(d, result->gradient ? &gradient : nullptr);
     const auto& o = p.options();
     const std::vector<double> h = p.hcore();
     std::vector<double> fock(matrix);
@@ -291,6 +296,10 @@ generativeqc_status generativeqc_fock_plan_evaluate(generativeqc_fock_plan* plan
     finite(fock);
     if (result->coulomb && o.coulomb) std::copy(jk.J.begin(), jk.J.end(), result->coulomb);
     if (result->exchange_alpha && o.exchange) std::copy(jk.K.begin(), jk.K.end(), result->exchange_alpha);
+    if (result->gradient) {
+      finite(gradient);
+      std::copy(gradient.begin(), gradient.end(), result->gradient);
+    }
     if (result->fock_alpha) std::copy(fock.begin(), fock.end(), result->fock_alpha);
     result->energy_one_electron = e1;
     result->energy_two_electron = e2;
@@ -313,11 +322,15 @@ generativeqc_status generativeqc_fock_plan_solve(generativeqc_fock_plan* plan,
     const uint64_t count = static_cast<uint64_t>(p.nbf()) * p.nbf();
     require(output->density ? output->density_count == count : output->density_count == 0,
             "SCF density output shape mismatch");
-    if (output->forces || output->force_count)
-      throw vqc::NotImplemented("analytic forces are not yet provided by this engine");
+    const uint64_t coordinates = static_cast<uint64_t>(p.system().atoms.size()) * 3;
+    require(output->forces ? output->force_count == coordinates && plan->resolved.derivative_order == 1
+                           : output->force_count == 0,
+            "SCF force output shape or derivative capability mismatch");
     require(initial_density ? initial_density_count == count : initial_density_count == 0,
             "SCF initial density shape mismatch");
-    disjoint_outputs({{output->density, buffer_bytes(output->density_count)}, {output, sizeof(*output)}});
+    disjoint_outputs({{output->density, buffer_bytes(output->density_count)},
+                      {output->forces, buffer_bytes(output->force_count)},
+                      {output, sizeof(*output)}});
     vqc::ScfOptions opts;
     if (controls) {
       require(controls->max_iterations > 0 && controls->diis_history > 0 &&
@@ -334,13 +347,15 @@ generativeqc_status generativeqc_fock_plan_solve(generativeqc_fock_plan* plan,
       seed.assign(initial_density, initial_density + count);
       finite(seed);
     }
-    const vqc::ScfOutcome r = p.solve(opts, initial_density ? &seed : nullptr);
+    const vqc::ScfOutcome r = p.solve(opts, initial_density ? &seed : nullptr, output->forces != nullptr);
     if (!r.converged) {
       plan->detail = "Fock SCF did not converge in " + std::to_string(r.iterations) + " iterations";
       return GENERATIVEQC_STATUS_NOT_CONVERGED;
     }
     finite(r.density);
+    finite(r.forces);
     if (output->density) std::copy(r.density.begin(), r.density.end(), output->density);
+    if (output->forces) std::copy(r.forces.begin(), r.forces.end(), output->forces);
     output->energy = r.energy;
     output->energy_change = r.energy_change;
     output->density_rms = r.density_rms;
@@ -379,10 +394,12 @@ generativeqc_status generativeqc_fock_plan_diagnostic(const generativeqc_fock_pl
   copy_text(d.df_value_backend, sizeof(d.df_value_backend), "hip-gfx950");
   copy_text(d.df_value_mapping, sizeof(d.df_value_mapping),
             p.cholesky() ? "cholesky-packed" : "eigen-cutoff-packed");
-  copy_text(d.df_response_mapping, sizeof(d.df_response_mapping), "none");
+  copy_text(d.df_response_mapping, sizeof(d.df_response_mapping),
+            plan->resolved.derivative_order ? "shell-pair-deterministic" : "absent");
   copy_text(d.one_electron_value_backend, sizeof(d.one_electron_value_backend), "hip-gfx950");
   copy_text(d.one_electron_value_mapping, sizeof(d.one_electron_value_mapping), "shell-pair");
-  copy_text(d.one_electron_response_mapping, sizeof(d.one_electron_response_mapping), "none");
+  copy_text(d.one_electron_response_mapping, sizeof(d.one_electron_response_mapping),
+            plan->resolved.derivative_order ? "pair-thread" : "absent");
   *diagnostic = d;
   return GENERATIVEQC_STATUS_SUCCESS;
 }

@@ -1,0 +1,10 @@
+// Nuclear-derivative kernels of the orbital class (l=2, l=0); compiled as its
+// own translation unit so the classes build in parallel.
+#include "integrals_grad.hpp"
+
+namespace vqc {
+namespace detail {
+VQC_DEFINE_GRAD_CLASS(2, 0)
+VQC_DEFINE_METRIC_GRAD(2)
+}  // namespace detail
+}  // namespace vqc

@@ -122,9 +122,10 @@ void DfPlan::setup(const System& aux) {
 
   Trace trace("plan_setup");
   boys_.upload(boys_table_host(), stream_);
-  DeviceBasis ob;
+  aux_ = aux;
+  DeviceBasis& ob = ob_;
   upload_basis(orb_, ob, false, stream_);
-  AuxiliaryLayout al;
+  AuxiliaryLayout& al = al_;
   build_auxiliary_layout(aux, al, stream_);
   trace.mark("basis_upload");
 
@@ -193,6 +194,7 @@ void DfPlan::factor_metric(DeviceArray<double>& M, DeviceArray<double>& A) {
                              rocblas_diagonal_non_unit, na, cols, &one, L.data(), na, A.data() + c0 * ldq_, ldq_));
     }
     sync();
+    L_ = std::move(L);  // kept for the gradient's c = L^-T rho and Y = L^-T T
     return;
   }
   // Eigen-cutoff route: X = V diag(lambda^-1/2 | lambda > thr lambda_max) V^T, B = A X.
@@ -359,7 +361,7 @@ int DfPlan::factorise_density(const double* D) {
   return static_cast<int>(pos.size()) | (r << 16);
 }
 
-JkResult DfPlan::evaluate(const std::vector<double>& density) {
+JkResult DfPlan::evaluate(const std::vector<double>& density, std::vector<double>* gradient) {
   VQC_HIP(hipSetDevice(device_));
   const std::size_t nn = static_cast<std::size_t>(n_) * n_;
   if (density.size() != nn) throw std::invalid_argument("density shape mismatch");
@@ -382,7 +384,7 @@ JkResult DfPlan::evaluate(const std::vector<double>& density) {
   D_.upload(sym.data(), nn, stream_);
   int packed = opts_.exchange ? factorise_density(D_.data()) : 0;
   const int r_pos = packed & 0xffff, r = packed >> 16;
-  build_jk(prec_, D_.data(), r, r_pos);
+  build_jk(gradient ? Precision::FP64 : prec_, D_.data(), r, r_pos);
   JkResult out;
   out.J.resize(nn);
   out.K.resize(nn);
@@ -391,6 +393,15 @@ JkResult DfPlan::evaluate(const std::vector<double>& density) {
   sync();
   if (!opts_.coulomb) std::fill(out.J.begin(), out.J.end(), 0.0);
   if (!opts_.exchange) std::fill(out.K.begin(), out.K.end(), 0.0);
+  if (gradient) {
+    const std::size_t ng = orb_.atoms.size() * 3;
+    DeviceArray<double> g(ng);
+    VQC_HIP(hipMemsetAsync(g.data(), 0, g.bytes(), stream_));
+    df_two_electron_gradient(r, r_pos, g.data());
+    gradient->assign(ng, 0.0);
+    g.download(gradient->data(), ng, stream_);
+    sync();
+  }
   return out;
 }
 
@@ -474,7 +485,7 @@ std::vector<double> DfPlan::overlap() const {
   return s;
 }
 
-ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial_density) {
+ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial_density, bool forces) {
   VQC_HIP(hipSetDevice(device_));
   if (orb_.multiplicity != 1 || orb_.electrons % 2 != 0)
     throw NotImplemented("only closed-shell restricted SCF is implemented");
@@ -636,6 +647,25 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   sync();
   out.energy = 0.5 * (hs[0] + hs[1]) + enuc_;
   out.converged = std::isfinite(out.energy);
+  if (forces && out.converged) {
+    // Complete gradient at the converged density: the last build above left
+    // rho_, the FP64 U_ and the factor sqrt(2) C_occ of D_ in place, and w
+    // holds the orbital energies of the same diagonalisation.
+    const std::size_t ng = orb_.atoms.size() * 3;
+    DeviceArray<double> g(ng);
+    VQC_HIP(hipMemsetAsync(g.data(), 0, g.bytes(), stream_));
+    one_electron_gradient(D_.data(), w.data(), nocc, g.data());
+    df_two_electron_gradient(nocc, nocc, g.data());
+    std::vector<double> hg(ng);
+    g.download(hg.data(), ng, stream_);
+    sync();
+    const auto gn = nuclear_repulsion_gradient();
+    out.forces.resize(ng);
+    for (std::size_t k = 0; k < ng; ++k) {
+      out.forces[k] = -(hg[k] + gn[k]);
+      if (!std::isfinite(out.forces[k])) throw NumericalFailure("non-finite analytic force");
+    }
+  }
   return out;
 }
 

@@ -35,6 +35,7 @@ struct ScfOutcome {
   unsigned reduced_iterations = 0;  // iterations run at reduced precision
   uint64_t fock_builds = 0;
   std::vector<double> density;
+  std::vector<double> forces;  // 3 natom, -dE/dR, when requested
 };
 
 struct JkResult {
@@ -65,14 +66,17 @@ class DfPlan {
   std::vector<double> hcore() const;
   std::vector<double> overlap() const;
 
-  // Raw J/K for a symmetric density (host in/out).
-  JkResult evaluate(const std::vector<double>& density);
+  // Raw J/K for a symmetric density (host in/out).  With `gradient` non-null
+  // the build runs in FP64 and *gradient receives the fixed-density
+  // two-electron energy derivative dE2/dR (3 natom, Hartree/bohr).
+  JkResult evaluate(const std::vector<double>& density, std::vector<double>* gradient = nullptr);
   // Raw J/K for D = 2 C C^T (C row-major n x nocc, host).
   JkResult evaluate_occupied(const std::vector<double>& occupied, int nocc);
   // Timing harness (see vibeqc_fock_plan_benchmark_occupied).
   std::array<double, 5> benchmark_occupied(const std::vector<double>& occupied, int nocc, unsigned repeat);
 
-  ScfOutcome solve(const ScfOptions& options, const std::vector<double>* initial_density);
+  // With forces, the outcome carries -dE/dR of the converged energy.
+  ScfOutcome solve(const ScfOptions& options, const std::vector<double>* initial_density, bool forces = false);
 
  private:
   void setup(const System& aux);
@@ -85,11 +89,17 @@ class DfPlan {
   void upload_factor(const std::vector<double>& occupied, int nocc);
   int factorise_density(const double* D);
   void sync();
+  // Gradient pieces (gradient.hip).  df_two_electron_gradient needs the state
+  // of an FP64 build of D_ with the factor in c64_ (rho_, FP64 U_); it
+  // overwrites U_.  All add into grad (device, 3 natom).
+  void df_two_electron_gradient(int r, int r_pos, double* grad);
+  void one_electron_gradient(const double* D, const double* eps, int nocc, double* grad);
+  std::vector<double> nuclear_repulsion_gradient() const;
 
   int device_;
   hipStream_t stream_ = nullptr;
   rocblas_handle blas_ = nullptr;
-  System orb_;
+  System orb_, aux_;
   PlanOptions opts_;
   Precision prec_ = Precision::FP64;
   int n_ = 0, naux_ = 0, ldq_ = 0, nocc_ = 0;
@@ -100,6 +110,9 @@ class DfPlan {
   double enuc_ = 0.0;
 
   DeviceArray<double> boys_;
+  DeviceBasis ob_;        // orbital basis as the integral kernels see it
+  AuxiliaryLayout al_;    // auxiliary functions in internal order
+  DeviceArray<double> L_;  // Cholesky factor of (P|Q), column-major lower (Cholesky path only)
   DeviceArray<double> S_, H_;
   DeviceArray<double> B64_;
   DeviceArray<float> B32_;

@@ -46,11 +46,12 @@ class FockTerm:
 
 @dataclass(frozen=True)
 class FockBuildSpec:
-    """J/K semantics (fock.py:56-101).  derivative_order 1 is not yet provided
-    by the HIP engine and is rejected at plan creation."""
+    """J/K semantics (fock.py:56-101).  derivative_order 1 (the default, as in
+    the reference) lets evaluate() return the fixed-density two-electron
+    gradient and solve() the complete forces."""
 
     spin: str = "restricted"
-    derivative_order: int = 0
+    derivative_order: int = 1
     coulomb: FockTerm = field(default_factory=FockTerm)
     exchange: FockTerm = field(default_factory=lambda: FockTerm(coefficient=-0.5))
 
@@ -59,7 +60,7 @@ class FockBuildSpec:
             raise ValueError("unsupported Fock spin/derivative order")
 
     @classmethod
-    def hf(cls, spin="restricted", *, coulomb="density_fitted", exchange="density_fitted", derivative_order=0):
+    def hf(cls, spin="restricted", *, coulomb="density_fitted", exchange="density_fitted", derivative_order=1):
         return cls(
             spin,
             derivative_order,
@@ -283,9 +284,13 @@ class FockPlan:
 
     # -- operations ----------------------------------------------------------
     def evaluate(self, density, *, derivative: bool = False) -> FockEvaluation:
-        """Raw J/K, Fock (with Hcore) and energies at a fixed symmetric density."""
-        if derivative:
-            raise NotImplementedError("analytic two-electron derivatives are not yet provided")
+        """Raw J/K, Fock (with Hcore) and energies at a fixed symmetric density.
+
+        With ``derivative=True`` (plan created with derivative_order 1) the
+        result's ``gradient`` holds the fixed-density two-electron energy
+        derivative dE2/dR, flat (3 natom), Hartree/bohr (fock.py:610-672)."""
+        if type(derivative) is not bool:
+            raise TypeError("derivative must be bool")
         n = self.nao
         d = np.array(density, dtype=np.float64, order="C", copy=True)
         if d.shape != (n, n) or not np.isfinite(d).all():
@@ -295,16 +300,17 @@ class FockPlan:
             j = np.empty((n, n)) if self.spec.coulomb.present else None
             k = np.empty((n, n)) if self.spec.exchange.present else None
             f = np.empty((n, n))
+            g = np.empty(self.basis.natom * 3) if derivative else None
             out = _native.descriptor(
                 _native.FockResultC,
                 matrix_count=n * n,
-                gradient_count=0,
+                gradient_count=0 if g is None else g.size,
                 coulomb=_native.pointer(j),
                 exchange_alpha=_native.pointer(k),
                 exchange_beta=None,
                 fock_alpha=_native.pointer(f),
                 fock_beta=None,
-                gradient=None,
+                gradient=_native.pointer(g),
             )
             status = self._lib.generativeqc_fock_plan_evaluate(self._handle, _native.pointer(d), n * n, None, 0, ct.byref(out))
             self._check(status)
@@ -315,6 +321,7 @@ class FockPlan:
                 _immutable(f),
                 _immutable(j) if j is not None else None,
                 _immutable(k) if k is not None else None,
+                _immutable(g) if g is not None else None,
             )
 
     def evaluate_occupied(self, occupied):
@@ -358,9 +365,12 @@ class FockPlan:
         energy_tolerance: float = 1e-10,
         density_tolerance: float = 1e-8,
     ) -> FockScfResult:
-        """Device-resident RHF SCF for the declared J/K energy (fock.py:674-776)."""
-        if compute_forces:
-            raise NotImplementedError("analytic forces are not yet provided")
+        """Device-resident RHF SCF for the declared J/K energy (fock.py:674-776).
+
+        ``compute_forces`` returns the complete forces -dE/dR, (natom, 3),
+        Hartree/bohr, at the converged density."""
+        if type(compute_forces) is not bool:
+            raise TypeError("compute_forces must be bool")
         n = self.nao
         seed = None
         if initial_density is not None:
@@ -375,7 +385,14 @@ class FockPlan:
             energy_tolerance=energy_tolerance,
             density_tolerance=density_tolerance,
         )
-        out = _native.descriptor(_native.ScfResultC, density=_native.pointer(density), forces=None, density_count=n * n, force_count=0)
+        forces = np.empty((self.basis.natom, 3)) if compute_forces else None
+        out = _native.descriptor(
+            _native.ScfResultC,
+            density=_native.pointer(density),
+            forces=_native.pointer(forces),
+            density_count=n * n,
+            force_count=0 if forces is None else forces.size,
+        )
         with self._lock:
             self._ensure_open()
             status = self._lib.generativeqc_fock_plan_solve(
@@ -385,7 +402,7 @@ class FockPlan:
         return FockScfResult(
             out.energy,
             _immutable(density),
-            None,
+            _immutable(forces) if forces is not None else None,
             out.iterations,
             out.energy_change,
             out.density_rms,

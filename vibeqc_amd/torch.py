@@ -1,11 +1,12 @@
 """PyTorch binding (host mirror of python/generativeqc/torch.py:47-245).
 
 ``energy(coordinates, atomic_numbers, calculator)`` returns the DF-RHF energy
-as a 0-d tensor in the input dtype/device.  ``batched_energy`` evaluates a list
-of molecules.  The backward needs analytic nuclear forces, which this engine
-does not provide yet: calling ``backward`` raises NotImplementedError rather
-than returning a silent or approximate gradient.  Higher-order derivatives are
-refused as in the reference (torch.py:34-44).
+as a 0-d tensor in the input dtype/device; its backward is -forces * grad_out
+with the analytic forces of the converged SCF (torch.py:47-84).
+``batched_energy`` evaluates a list of molecules.  ``density_energy(plan, D)``
+is the fixed-density energy 1/2 tr D (H + F(D)) + E_nuc of a prepared plan with
+the backward dE/dD = F (the check of test_fock.py:56-60).  Higher-order
+derivatives are refused as in the reference (torch.py:34-44).
 """
 
 from __future__ import annotations
@@ -25,13 +26,43 @@ class _EnergyFunction(torch.autograd.Function):
     def forward(ctx, coordinates, atomic_numbers, calculator, charge, multiplicity):
         values = coordinates.detach().cpu().double()
         atoms = [(int(atomic_numbers[i]), values[i].tolist()) for i in range(coordinates.shape[0])]
-        result = calculator.singlepoint(atoms, charge=charge, multiplicity=multiplicity, properties=("energy",))
+        want = ("energy", "forces") if coordinates.requires_grad else ("energy",)
+        result = calculator.singlepoint(atoms, charge=charge, multiplicity=multiplicity, properties=want)
+        if result.forces is not None:
+            ctx.save_for_backward(torch.as_tensor(result.forces, dtype=coordinates.dtype, device=coordinates.device))
         return coordinates.new_tensor(result.energy)
 
     @staticmethod
     def backward(ctx, grad_output):
         _require_first_order_backward()
-        raise NotImplementedError("analytic DF-RHF forces are not yet provided by the HIP engine")
+        (forces,) = ctx.saved_tensors
+        # Forces are -dE/dR; autograd wants dE/dR.
+        return -forces * grad_output, None, None, None, None
+
+
+class _DensityEnergyFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, density, plan):
+        d = density.detach().cpu().double().numpy()
+        ev = plan.evaluate(0.5 * (d + d.T))
+        ctx.save_for_backward(torch.as_tensor(ev.fock, dtype=density.dtype, device=density.device))
+        return density.new_tensor(ev.energy)
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        _require_first_order_backward()
+        (fock,) = ctx.saved_tensors
+        return fock * grad_output, None
+
+
+def density_energy(plan, density):
+    """E(D) = 1/2 tr D (H + F(D)) + E_nuc at a fixed symmetric density, with
+    dE/dD = F(D) as its backward."""
+    if not isinstance(density, torch.Tensor) or not density.is_floating_point():
+        raise TypeError("density must be a real floating-point tensor")
+    if density.shape != (plan.nao, plan.nao):
+        raise ValueError("density must be nbf x nbf")
+    return _DensityEnergyFunction.apply(density, plan)
 
 
 def _validated(coordinates, atomic_numbers):
