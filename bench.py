@@ -257,6 +257,78 @@ def _cpu_baseline(n, naux, nocc, fraction, seed=7):
     }
 
 
+def _ragged_systems(count=512, seed=2):
+    """Ragged batch: (H2O)n with n ~ U{1..16} from a seed, each in a cube of
+    side 3 n^(1/3) A (rigid monomers, O-O >= 2.6 A)."""
+    from vibeqc_amd.geometry import water_cluster
+
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(1, 17, size=count)
+    return [water_cluster(int(n), 3.0 * float(n) ** (1.0 / 3.0), seed=1000 + k) for k, n in enumerate(sizes)], sizes
+
+
+def _ragged_batch(count=512):
+    from vibeqc_amd import Calculator
+
+    systems, sizes = _ragged_systems(count)
+    out = {"molecules": count, "sizes_histogram": np.bincount(sizes, minlength=17)[1:].tolist()}
+    energies = {}
+    for precision in ("mixed", "fp64"):
+        calc = Calculator(method="rhf", basis="cc-pvdz", auxiliary_basis="cc-pvdz-jkfit", precision=precision)
+        t0 = time.perf_counter()
+        with calc.prepare_batch(systems, warm_start=False) as batch:
+            t1 = time.perf_counter()
+            res = batch.execute(properties=("energy",))
+            t2 = time.perf_counter()
+        energies[precision] = res.energies
+        out[precision] = {
+            "molecules_per_s": count / (t2 - t0),
+            "prepare_s": t1 - t0,
+            "execute_s": t2 - t1,
+            "failures": len(res.failure_indices),
+            "buckets": len({it.bucket_id for it in res.items}),
+            "mean_iterations": float(np.mean([it.iterations for it in res.items])),
+        }
+        _log(f"ragged batch {precision}: {out[precision]}")
+    err = np.abs(energies["mixed"] - energies["fp64"])
+    out["max_abs_energy_error_mixed_vs_fp64"] = float(np.nanmax(err))
+    out["mean_abs_energy_error_mixed_vs_fp64"] = float(np.nanmean(err))
+    return out
+
+
+def _ragged_cpu_baseline(sample=4, max_waters=6):
+    """The oracle's FP64 DF-RHF (C integrals + numpy SCF) on the first
+    `sample` members of the same ragged batch with at most `max_waters`
+    waters (larger members take minutes each on the host)."""
+    from oracle import oracle as O
+
+    all_systems, all_sizes = _ragged_systems()
+    pick = [k for k, n in enumerate(all_sizes) if n <= max_waters][:sample]
+    systems = [all_systems[k] for k in pick]
+    sizes = all_sizes[pick]
+    from vibeqc_amd import shells_for
+
+    def oshells(shells):
+        return [O.Shell(s.atom_index, s.l, tuple(p.exponent for p in s.primitives), tuple(p.coefficient for p in s.primitives)) for s in shells]
+
+    t0 = time.perf_counter()
+    for atoms in systems:
+        z = [a.atomic_number for a in atoms]
+        xyz = np.array([a.position for a in atoms])
+        problem = O.build_problem(z, xyz, oshells(shells_for("cc-pvdz", atoms)), True,
+                                  oshells(shells_for("cc-pvdz-jkfit", atoms)), True)
+        O.df_rhf(problem)
+    dt = time.perf_counter() - t0
+    return {
+        "value": sample / dt,
+        "unit": "molecules/s",
+        "cores": os.cpu_count() or 1,
+        "kind": "port",
+        "sample": f"oracle DF-RHF (C integrals, numpy SCF) on batch members {pick} "
+        f"(the first {sample} with <= {max_waters} waters; sizes {sizes.tolist()})",
+    }
+
+
 def _scf_timing(precision):
     from vibeqc_amd import AOBasis, FockPlan, shells_for
     from vibeqc_amd.geometry import water_cluster
@@ -298,6 +370,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 PMC byte-count runs")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-molecule ragged batch")
     ap.add_argument(
         "--only",
         choices=("c20h42", "water64"),
@@ -345,6 +418,12 @@ def main():
         _log(f"cpu baseline (H2O)64: {cpu64}")
     scf64 = _scf_timing("fp64")
     scfmx = _scf_timing("bf16x3")
+    ragged = None
+    if not args.no_batch:
+        ragged = _ragged_batch()
+        if not args.no_cpu_baseline:
+            ragged["cpu_baseline"] = _ragged_cpu_baseline()
+            _log(f"ragged cpu baseline: {ragged['cpu_baseline']}")
     traffic = traffic64 = None
     if not args.no_traffic:
         traffic = _pmc_traffic("c20h42", "bf16x3")
@@ -392,6 +471,7 @@ def main():
             "roofline_fp64": _roofline(w64["fp64"]["phase_ms"], wn, wnaux, wnpair, wnocc, "fp64"),
             "cpu_baseline": cpu64,
         },
+        "ragged_batch": ragged,
         "scf_water4": {
             "fp64": scf64,
             "mixed": scfmx,

@@ -198,6 +198,72 @@ class FockDiagnosticC(ct.Structure):
     ]
 
 
+METHOD_RHF = 1
+DENSITY_FITTING_CUDA = 2
+PRECISION_MODE = {"fp64": 0, "auto": 1}
+BATCH_WARM_STARTS = 1
+
+
+class MethodDescriptorC(ct.Structure):
+    """generativeqc_method_descriptor (reference generativeqc.h:723-769)."""
+
+    _fields_ = [
+        ("struct_size", ct.c_uint32),
+        ("abi_version", ct.c_uint32),
+        ("method", ct.c_int32),
+        ("max_iterations", ct.c_uint32),
+        ("diis_history", ct.c_uint32),
+        ("energy_tolerance", ct.c_double),
+        ("density_tolerance", ct.c_double),
+        ("screening_tolerance", ct.c_double),
+        ("density_fitting_mode", ct.c_int32),
+        ("density_fitting_auxiliary_basis", ct.c_void_p),
+        ("density_fitting_relative_threshold", ct.c_double),
+        ("density_fitting_memory_budget_bytes", ct.c_uint64),
+        ("precision_mode", ct.c_int32),
+        ("correlation_memory_budget_bytes", ct.c_uint64),
+        ("mp2_denominator_threshold", ct.c_double),
+        ("ks_options", ct.c_void_p),
+        ("ccsd_max_iterations", ct.c_uint32),
+        ("ccsd_diis_history", ct.c_uint32),
+        ("ccsd_energy_tolerance", ct.c_double),
+        ("ccsd_residual_tolerance", ct.c_double),
+        ("ccsd_denominator_threshold", ct.c_double),
+        ("ccsd_damping", ct.c_double),
+        ("ccsd_level_shift", ct.c_double),
+        ("ccsd_frozen_core", ct.c_uint32),
+        ("initial_guess", ct.c_void_p),
+    ]
+
+
+class BatchInputC(ct.Structure):
+    _fields_ = [
+        ("struct_size", ct.c_uint32),
+        ("abi_version", ct.c_uint32),
+        ("coordinates", _DP),
+        ("coordinate_count", ct.c_uint32),
+    ]
+
+
+class BatchItemResultC(ct.Structure):
+    _fields_ = [
+        ("struct_size", ct.c_uint32),
+        ("abi_version", ct.c_uint32),
+        ("status", ct.c_int32),
+        ("energy", ct.c_double),
+        ("forces", _DP),
+        ("force_count", ct.c_uint32),
+        ("iterations", ct.c_uint32),
+        ("energy_change", ct.c_double),
+        ("density_rms", ct.c_double),
+        ("converged", ct.c_int32),
+        ("executed_backend", ct.c_int32),
+        ("bucket_id", ct.c_uint32),
+        ("warm_start_used", ct.c_int32),
+        ("warm_start_fallback", ct.c_int32),
+    ]
+
+
 def descriptor(kind, **values):
     """A versioned descriptor with struct_size / abi_version filled in."""
     return kind(struct_size=ct.sizeof(kind), abi_version=ABI_VERSION, **values)
@@ -236,6 +302,31 @@ EXPORTS = {
     ),
     "vibeqc_fock_plan_shape": (ct.c_int32, [ct.c_void_p, ct.POINTER(ct.c_uint64)]),
     "vibeqc_fock_plan_one_electron": (ct.c_int32, [ct.c_void_p, _DP, _DP, ct.c_uint64]),
+    "generativeqc_batch_prepare": (
+        ct.c_int32,
+        [ct.c_void_p, ct.POINTER(ct.c_void_p), ct.c_uint32, ct.POINTER(MethodDescriptorC), ct.c_uint32, ct.POINTER(ct.c_void_p)],
+    ),
+    "vibeqc_batch_prepare": (
+        ct.c_int32,
+        [
+            ct.c_void_p,
+            ct.POINTER(ct.c_void_p),
+            ct.POINTER(ct.c_void_p),
+            ct.c_uint32,
+            ct.POINTER(MethodDescriptorC),
+            ct.c_uint32,
+            ct.POINTER(ct.c_void_p),
+        ],
+    ),
+    "generativeqc_batch_destroy": (None, [ct.c_void_p]),
+    "generativeqc_batch_get_system_count": (ct.c_uint32, [ct.c_void_p]),
+    "generativeqc_batch_execute": (
+        ct.c_int32,
+        [ct.c_void_p, ct.POINTER(BatchInputC), ct.c_uint32, ct.POINTER(BatchItemResultC), ct.c_uint32],
+    ),
+    "generativeqc_batch_set_warm_start_updates": (ct.c_int32, [ct.c_void_p, ct.c_int32]),
+    "generativeqc_batch_get_last_fock_builds": (ct.c_int32, [ct.c_void_p, ct.c_uint32, ct.POINTER(ct.c_uint64)]),
+    "vibeqc_batch_bucket": (ct.c_int32, [ct.c_void_p, ct.c_uint32, ct.POINTER(ct.c_uint32)]),
 }
 
 

@@ -113,6 +113,38 @@ class Calculator:
         )
         return orbital, aux
 
+    def _method_descriptor(self):
+        """generativeqc_method_descriptor of this calculator (generativeqc.h:723-769)."""
+        from . import _native
+
+        if self.precision not in ("fp64", "mixed", "bf16x3"):
+            raise NotImplementedError("batches run in fp64 or mixed (bf16x3 + FP64 refinement) precision")
+        return _native.descriptor(
+            _native.MethodDescriptorC,
+            method=_native.METHOD_RHF,
+            max_iterations=self.max_iterations,
+            diis_history=self.diis_history,
+            energy_tolerance=self.energy_tolerance,
+            density_tolerance=self.density_tolerance,
+            screening_tolerance=self.screening_tolerance,
+            density_fitting_mode=_native.DENSITY_FITTING_CUDA,
+            density_fitting_auxiliary_basis=None,
+            density_fitting_relative_threshold=self.metric_threshold,
+            density_fitting_memory_budget_bytes=self.budget,
+            precision_mode=_native.PRECISION_MODE["fp64" if self.precision == "fp64" else "auto"],
+        )
+
+    def prepare_batch(self, systems, *, charges=None, multiplicities=None, warm_start: bool = True):
+        """Prepare a persistent ragged batch (calculator.py:1931-1963)."""
+        from .batch import PreparedBatch
+
+        return PreparedBatch(self, systems, charges=charges, multiplicities=multiplicities, warm_start=warm_start)
+
+    def batch_singlepoint(self, systems, *, charges=None, multiplicities=None, strict: bool = False, properties=None):
+        """One-shot ragged batch with per-system status (calculator.py:1965-1980)."""
+        with self.prepare_batch(systems, charges=charges, multiplicities=multiplicities, warm_start=False) as batch:
+            return batch.execute(strict=strict, properties=properties)
+
     def prepare(self, atoms, *, charge: int = 0, multiplicity: int = 1) -> FockPlan:
         """Build the device plan (integrals, metric factor, B) for one geometry."""
         atoms = _normalise_atoms(atoms)

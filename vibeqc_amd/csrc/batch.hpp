@@ -1,0 +1,62 @@
+// Ragged batch ("fleet") of independent closed-shell DF-RHF systems on one
+// HIP device: the engine behind generativeqc_batch_prepare / _execute
+// (reference generativeqc.h:1431-1586, src/scf/fleet.cpp).
+//
+// Systems are grouped into shape buckets (nbf, naux, nocc), as the reference
+// groups its fleets (fleet.cpp:26-36), and executed largest first by a pool of
+// host workers.  Each worker owns one HIP stream and one rocBLAS handle and
+// allocates stream-ordered, so the many small plans of a ragged batch run
+// concurrently on the GPU instead of one after another.  A plan lives only
+// while its system is solved, which bounds device memory by the largest few
+// systems rather than the whole batch.
+#pragma once
+
+#include <exception>
+#include <memory>
+#include <vector>
+
+#include "plan.hpp"
+
+namespace vqc {
+
+struct BatchOptions {
+  PlanOptions plan;
+  ScfOptions scf;
+  Precision precision = Precision::FP64;
+  bool warm_starts = false;
+  int workers = 0;  // 0: VIBEQC_BATCH_WORKERS if set, else 8 (never more than the systems)
+};
+
+struct BatchItemOutcome {
+  std::exception_ptr error;  // set when the item failed
+  ScfOutcome scf;
+  int bucket = 0;
+  bool warm_start_used = false;
+  bool warm_start_fallback = false;
+};
+
+class BatchPlan {
+ public:
+  // aux[i] is the auxiliary basis of systems[i] (same atoms).
+  BatchPlan(int device, std::vector<System> systems, std::vector<System> aux, const BatchOptions& options);
+  std::size_t size() const { return systems_.size(); }
+  int bucket_of(std::size_t i) const { return bucket_[i]; }
+  int bucket_count() const { return buckets_; }
+  void set_warm_updates(bool enabled) { warm_updates_ = enabled; }
+  // coordinates[i]: replacement flat xyz (Bohr) for system i, or null.
+  // forces[i]: whether item i reports forces.
+  std::vector<BatchItemOutcome> execute(const std::vector<const std::vector<double>*>& coordinates,
+                                        const std::vector<bool>& forces);
+
+ private:
+  int device_;
+  std::vector<System> systems_, aux_;
+  BatchOptions opts_;
+  std::vector<int> bucket_;
+  int buckets_ = 0;
+  std::vector<std::size_t> order_;  // most expensive first
+  std::vector<std::vector<double>> warm_;
+  bool warm_updates_ = true;
+};
+
+}  // namespace vqc

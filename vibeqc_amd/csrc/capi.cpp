@@ -16,7 +16,9 @@
 #include <vector>
 
 #include "basis.hpp"
+#include "batch.hpp"
 #include "common.hpp"
+#include "generativeqc/batch.h"
 #include "generativeqc/fock.h"
 #include "generativeqc/generativeqc.h"
 #include "plan.hpp"
@@ -29,6 +31,13 @@ struct generativeqc_context {
 
 struct generativeqc_system {
   vqc::System data;
+};
+
+struct generativeqc_batch {
+  std::unique_ptr<vqc::BatchPlan> plan;
+  std::vector<std::size_t> natoms;
+  std::vector<uint64_t> fock_builds;
+  std::string detail;
 };
 
 struct generativeqc_fock_plan {
@@ -112,6 +121,79 @@ bool term_ok(const generativeqc_fock_term& t) {
 void copy_text(char* dst, std::size_t cap, const char* src) {
   std::strncpy(dst, src, cap - 1);
   dst[cap - 1] = '\0';
+}
+
+// RHF/DF options of a method descriptor (reference generativeqc.h:723-769).
+vqc::BatchOptions batch_options(const generativeqc_method_descriptor& d, generativeqc_batch_flags flags) {
+  if (d.method != GENERATIVEQC_METHOD_RHF) throw vqc::NotImplemented("batches implement the RHF method only");
+  if (d.density_fitting_mode != GENERATIVEQC_DENSITY_FITTING_CUDA &&
+      d.density_fitting_mode != GENERATIVEQC_DENSITY_FITTING_AUTO)
+    throw vqc::NotImplemented("batches run density-fitted J/K on the HIP device only");
+  if (d.ks_options || d.initial_guess) throw vqc::NotImplemented("KS and initial-guess options are not implemented");
+  require(d.precision_mode == GENERATIVEQC_PRECISION_FP64 || d.precision_mode == GENERATIVEQC_PRECISION_AUTO,
+          "unknown precision mode");
+  auto nonneg = [](double v) { return std::isfinite(v) && v >= 0.0; };
+  require(nonneg(d.energy_tolerance) && nonneg(d.density_tolerance) && nonneg(d.screening_tolerance) &&
+              nonneg(d.density_fitting_relative_threshold) && d.density_fitting_relative_threshold < 1.0,
+          "method tolerances must be finite and nonnegative");
+  vqc::BatchOptions o;
+  o.plan.screening = d.screening_tolerance > 0.0 ? d.screening_tolerance : 1e-12;
+  o.plan.metric_threshold = d.density_fitting_relative_threshold > 0.0 ? d.density_fitting_relative_threshold : 1e-10;
+  if (d.max_iterations) o.scf.max_iterations = d.max_iterations;
+  if (d.diis_history) o.scf.diis_history = std::min<uint32_t>(d.diis_history, 32);
+  if (d.energy_tolerance > 0.0) o.scf.energy_tolerance = d.energy_tolerance;
+  if (d.density_tolerance > 0.0) o.scf.density_tolerance = d.density_tolerance;
+  o.precision = d.precision_mode == GENERATIVEQC_PRECISION_AUTO ? vqc::Precision::BF16X3 : vqc::Precision::FP64;
+  o.warm_starts = (flags & GENERATIVEQC_BATCH_ENABLE_WARM_STARTS) != 0;
+  return o;
+}
+
+// The template's shells placed on another system's atoms (same atom count).
+vqc::System auxiliary_from_template(const vqc::System& tmpl, const vqc::System& sys) {
+  if (tmpl.atoms.size() != sys.atoms.size())
+    throw std::invalid_argument("auxiliary template atom topology differs from a batch system");
+  vqc::System aux = tmpl;
+  aux.atoms = sys.atoms;
+  aux.charge = sys.charge;
+  aux.multiplicity = sys.multiplicity;
+  aux.electrons = sys.electrons;
+  return aux;
+}
+
+generativeqc_status prepare_batch(generativeqc_context* context, const generativeqc_system* const* systems,
+                                  const generativeqc_system* const* auxiliary, uint32_t count,
+                                  const generativeqc_method_descriptor* descriptor, generativeqc_batch_flags flags,
+                                  generativeqc_batch** batch) {
+  if (batch) *batch = nullptr;
+  if (!context || !systems || !descriptor || !batch || count == 0) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  if (!valid_descriptor(descriptor)) return GENERATIVEQC_STATUS_ABI_MISMATCH;
+  std::lock_guard<std::recursive_mutex> lock(context->mutex);
+  try {
+    const vqc::BatchOptions opts = batch_options(*descriptor, flags);
+    std::vector<vqc::System> orb, aux;
+    orb.reserve(count);
+    aux.reserve(count);
+    for (uint32_t i = 0; i < count; ++i) {
+      require(systems[i] != nullptr, "null batch system");
+      orb.push_back(systems[i]->data);
+      if (auxiliary) {
+        require(auxiliary[i] != nullptr, "null auxiliary system");
+        aux.push_back(auxiliary_from_template(auxiliary[i]->data, orb.back()));
+      } else if (descriptor->density_fitting_auxiliary_basis) {
+        aux.push_back(auxiliary_from_template(descriptor->density_fitting_auxiliary_basis->data, orb.back()));
+      } else {
+        aux.push_back(orb.back());
+      }
+    }
+    auto out = std::make_unique<generativeqc_batch>();
+    for (const auto& s : orb) out->natoms.push_back(s.atoms.size());
+    out->fock_builds.assign(count, 0);
+    out->plan = std::make_unique<vqc::BatchPlan>(context->device, std::move(orb), std::move(aux), opts);
+    *batch = out.release();
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(&context->detail);
+  }
 }
 
 }  // namespace
@@ -489,6 +571,112 @@ generativeqc_status vibeqc_fock_plan_shape(const generativeqc_fock_plan* plan, u
   shape6[4] = p.cholesky() ? 1 : 0;
   shape6[5] = static_cast<uint64_t>(p.device_bytes());
   return GENERATIVEQC_STATUS_SUCCESS;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+generativeqc_status generativeqc_batch_prepare(generativeqc_context* context, const generativeqc_system* const* systems,
+                                               uint32_t system_count, const generativeqc_method_descriptor* descriptor,
+                                               generativeqc_batch_flags flags, generativeqc_batch** batch) {
+  return prepare_batch(context, systems, nullptr, system_count, descriptor, flags, batch);
+}
+
+generativeqc_status vibeqc_batch_prepare(generativeqc_context* context, const generativeqc_system* const* systems,
+                                         const generativeqc_system* const* auxiliary_systems, uint32_t system_count,
+                                         const generativeqc_method_descriptor* descriptor,
+                                         generativeqc_batch_flags flags, generativeqc_batch** batch) {
+  return prepare_batch(context, systems, auxiliary_systems, system_count, descriptor, flags, batch);
+}
+
+void generativeqc_batch_destroy(generativeqc_batch* batch) { delete batch; }
+
+uint32_t generativeqc_batch_get_system_count(const generativeqc_batch* batch) {
+  return batch ? static_cast<uint32_t>(batch->plan->size()) : 0u;
+}
+
+generativeqc_status generativeqc_batch_set_warm_start_updates(generativeqc_batch* batch, int32_t enabled) {
+  if (!batch || (enabled != 0 && enabled != 1)) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  batch->plan->set_warm_updates(enabled != 0);
+  return GENERATIVEQC_STATUS_SUCCESS;
+}
+
+generativeqc_status generativeqc_batch_get_last_fock_builds(const generativeqc_batch* batch, uint32_t index,
+                                                            uint64_t* builds) {
+  if (!batch || !builds || index >= batch->fock_builds.size()) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  *builds = batch->fock_builds[index];
+  return GENERATIVEQC_STATUS_SUCCESS;
+}
+
+generativeqc_status vibeqc_batch_bucket(const generativeqc_batch* batch, uint32_t index, uint32_t* bucket_id) {
+  if (!batch || !bucket_id || index >= batch->plan->size()) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  *bucket_id = static_cast<uint32_t>(batch->plan->bucket_of(index));
+  return GENERATIVEQC_STATUS_SUCCESS;
+}
+
+generativeqc_status generativeqc_batch_execute(generativeqc_batch* batch, const generativeqc_batch_input_descriptor* inputs,
+                                               uint32_t input_count, generativeqc_batch_item_result_descriptor* results,
+                                               uint32_t result_count) {
+  if (!batch || !results) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  const std::size_t count = batch->plan->size();
+  batch->detail.clear();
+  try {
+    require(result_count == count, "batch result count mismatch");
+    require(inputs ? input_count == count : input_count == 0, "batch input count mismatch");
+    for (std::size_t i = 0; i < count; ++i) {
+      if (!valid_descriptor(&results[i]) || (inputs && !valid_descriptor(&inputs[i])))
+        return GENERATIVEQC_STATUS_ABI_MISMATCH;
+      const uint64_t coords = batch->natoms[i] * 3;
+      require(results[i].forces ? results[i].force_count == coords : results[i].force_count == 0,
+              "batch force output shape mismatch");
+      if (inputs) require(inputs[i].coordinates ? inputs[i].coordinate_count == coords : inputs[i].coordinate_count == 0,
+                          "batch coordinate shape mismatch");
+    }
+    std::vector<std::vector<double>> xyz(inputs ? count : 0);
+    std::vector<const std::vector<double>*> coordinates(inputs ? count : 0, nullptr);
+    for (std::size_t i = 0; inputs && i < count; ++i)
+      if (inputs[i].coordinates) {
+        xyz[i].assign(inputs[i].coordinates, inputs[i].coordinates + inputs[i].coordinate_count);
+        coordinates[i] = &xyz[i];
+      }
+    std::vector<bool> forces(count);
+    for (std::size_t i = 0; i < count; ++i) forces[i] = results[i].forces != nullptr;
+    const auto outcomes = batch->plan->execute(coordinates, forces);
+    for (std::size_t i = 0; i < count; ++i) {
+      auto& r = results[i];
+      const auto& o = outcomes[i];
+      r.bucket_id = static_cast<uint32_t>(o.bucket);
+      r.executed_backend = GENERATIVEQC_BACKEND_CUDA;
+      r.warm_start_used = o.warm_start_used;
+      r.warm_start_fallback = o.warm_start_fallback;
+      batch->fock_builds[i] = o.scf.fock_builds;
+      if (o.error) {
+        std::string ignored;
+        try {
+          std::rethrow_exception(o.error);
+        } catch (...) {
+          r.status = map_exception(&ignored);
+        }
+        r.converged = 0;
+        continue;
+      }
+      r.iterations = o.scf.iterations;
+      r.energy_change = o.scf.energy_change;
+      r.density_rms = o.scf.density_rms;
+      r.converged = o.scf.converged ? 1 : 0;
+      if (!o.scf.converged) {
+        r.status = GENERATIVEQC_STATUS_NOT_CONVERGED;
+        continue;
+      }
+      r.status = GENERATIVEQC_STATUS_SUCCESS;
+      r.energy = o.scf.energy;
+      if (r.forces) std::copy(o.scf.forces.begin(), o.scf.forces.end(), r.forces);
+    }
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(&batch->detail);
+  }
 }
 
 }  // extern "C"

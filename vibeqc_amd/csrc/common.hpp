@@ -6,8 +6,11 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <cstdint>
+#include <cstdlib>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -43,6 +46,76 @@ inline void blas_check(rocblas_status s, const char* what) {
 #define VQC_BLAS(call) ::vqc::blas_check((call), #call)
 #define VQC_LAUNCH(what) ::vqc::hip_check(hipGetLastError(), what)
 
+// Host<->device copies go through a pinned staging buffer owned by the
+// calling host thread and complete before returning, so no copy depends on
+// the runtime's shared pageable-memory staging or outlives its host buffer
+// (plans of a ragged batch copy from many host threads at once).
+class PinnedStaging {
+ public:
+  ~PinnedStaging() {
+    if (p_) (void)hipHostFree(p_);
+  }
+  void* get(std::size_t bytes) {
+    if (bytes > cap_) {
+      if (p_) (void)hipHostFree(p_);
+      p_ = nullptr;
+      cap_ = 0;
+      const std::size_t want = std::max<std::size_t>(bytes, std::size_t{1} << 20);
+      hip_check(hipHostMalloc(&p_, want, hipHostMallocDefault), "hipHostMalloc");
+      cap_ = want;
+    }
+    return p_;
+  }
+
+ private:
+  void* p_ = nullptr;
+  std::size_t cap_ = 0;
+};
+inline PinnedStaging& thread_staging() {
+  static thread_local PinnedStaging staging;
+  return staging;
+}
+// Larger transfers are moved in pieces of at most this many bytes.
+constexpr std::size_t kStagingChunk = std::size_t{64} << 20;
+inline void copy_to_device(void* dst, const void* src, std::size_t bytes, hipStream_t s) {
+  for (std::size_t off = 0; off < bytes; off += kStagingChunk) {
+    const std::size_t len = std::min(kStagingChunk, bytes - off);
+    void* stage = thread_staging().get(len);
+    std::memcpy(stage, static_cast<const char*>(src) + off, len);
+    hip_check(hipMemcpyAsync(static_cast<char*>(dst) + off, stage, len, hipMemcpyHostToDevice, s), "upload");
+    hip_check(hipStreamSynchronize(s), "upload");
+  }
+}
+inline void copy_to_host(void* dst, const void* src, std::size_t bytes, hipStream_t s) {
+  for (std::size_t off = 0; off < bytes; off += kStagingChunk) {
+    const std::size_t len = std::min(kStagingChunk, bytes - off);
+    void* stage = thread_staging().get(len);
+    hip_check(hipMemcpyAsync(stage, static_cast<const char*>(src) + off, len, hipMemcpyDeviceToHost, s), "download");
+    hip_check(hipStreamSynchronize(s), "download");
+    std::memcpy(static_cast<char*>(dst) + off, stage, len);
+  }
+}
+
+// Debug aid: VIBEQC_POISON_ALLOC=1 fills every new device allocation with
+// 0xFF bytes (NaN in FP64/FP32/BF16), so results that depend on memory read
+// before it is written become non-finite instead of silently varying.
+inline bool poison_allocations() {
+  static const bool on = [] {
+    const char* v = std::getenv("VIBEQC_POISON_ALLOC");
+    return v && v[0] == '1';
+  }();
+  return on;
+}
+
+// Stream used for stream-ordered allocation by DeviceArrays created on this
+// host thread (null: plain hipMalloc / hipFree).  Batch workers set it to
+// their own stream, so the many small plans they create and destroy never
+// hit hipFree's device-wide synchronisation and overlap on the GPU.
+inline hipStream_t& allocation_stream() {
+  static thread_local hipStream_t stream = nullptr;
+  return stream;
+}
+
 // Owned device allocation; moves, never copies.
 template <class T>
 class DeviceArray {
@@ -51,12 +124,14 @@ class DeviceArray {
   explicit DeviceArray(std::size_t n) { resize(n); }
   DeviceArray(const DeviceArray&) = delete;
   DeviceArray& operator=(const DeviceArray&) = delete;
-  DeviceArray(DeviceArray&& o) noexcept : p_(std::exchange(o.p_, nullptr)), n_(std::exchange(o.n_, 0)) {}
+  DeviceArray(DeviceArray&& o) noexcept
+      : p_(std::exchange(o.p_, nullptr)), n_(std::exchange(o.n_, 0)), s_(std::exchange(o.s_, nullptr)) {}
   DeviceArray& operator=(DeviceArray&& o) noexcept {
     if (this != &o) {
       release();
       p_ = std::exchange(o.p_, nullptr);
       n_ = std::exchange(o.n_, 0);
+      s_ = std::exchange(o.s_, nullptr);
     }
     return *this;
   }
@@ -66,35 +141,48 @@ class DeviceArray {
     release();
     if (n) {
       void* q = nullptr;
-      hip_check(hipMalloc(&q, n * sizeof(T)), "hipMalloc");
+      s_ = allocation_stream();
+      if (s_) hip_check(hipMallocAsync(&q, n * sizeof(T), s_), "hipMallocAsync");
+      else hip_check(hipMalloc(&q, n * sizeof(T)), "hipMalloc");
       p_ = static_cast<T*>(q);
       n_ = n;
+      if (poison_allocations()) {
+        hip_check(hipMemsetAsync(q, 0xFF, n * sizeof(T), s_), "poison");
+        hip_check(hipStreamSynchronize(s_), "poison");
+      }
     }
   }
   void release() {
-    if (p_) (void)hipFree(p_);
+    if (p_) {
+      if (s_) (void)hipFreeAsync(p_, s_);
+      else (void)hipFree(p_);
+    }
     p_ = nullptr;
     n_ = 0;
+    s_ = nullptr;
   }
   T* data() const { return p_; }
   std::size_t size() const { return n_; }
   std::size_t bytes() const { return n_ * sizeof(T); }
+  // Host-to-device copy, complete on return (the source may be a temporary).
   void upload(const T* host, std::size_t n, hipStream_t s) {
     if (n > n_) throw std::invalid_argument("upload larger than device buffer");
-    hip_check(hipMemcpyAsync(p_, host, n * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+    copy_to_device(p_, host, n * sizeof(T), s);
   }
   void upload(const std::vector<T>& v, hipStream_t s) {
     resize(v.size());
     upload(v.data(), v.size(), s);
   }
+  // Device-to-host copy, complete on return (orders after prior work on s).
   void download(T* host, std::size_t n, hipStream_t s) const {
     if (n > n_) throw std::invalid_argument("download larger than device buffer");
-    hip_check(hipMemcpyAsync(host, p_, n * sizeof(T), hipMemcpyDeviceToHost, s), "download");
+    copy_to_host(host, p_, n * sizeof(T), s);
   }
 
  private:
   T* p_ = nullptr;
   std::size_t n_ = 0;
+  hipStream_t s_ = nullptr;  // stream of a stream-ordered allocation
 };
 
 inline std::size_t pair_count(std::size_t n) { return n * (n + 1) / 2; }

@@ -78,17 +78,29 @@ void check_basis_limits(const System& sys, int lmax, const char* what) {
 
 }  // namespace
 
-DfPlan::DfPlan(int device, const System& orbital, const System& auxiliary, const PlanOptions& options)
+DfPlan::DfPlan(int device, const System& orbital, const System& auxiliary, const PlanOptions& options,
+               hipStream_t stream, rocblas_handle blas)
     : device_(device), orb_(orbital), opts_(options) {
   VQC_HIP(hipSetDevice(device_));
-  VQC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  VQC_BLAS(rocblas_create_handle(&blas_));
-  VQC_BLAS(rocblas_set_stream(blas_, stream_));
+  if (stream && blas) {
+    stream_ = stream;
+    blas_ = blas;
+    owns_stream_ = false;
+  } else {
+    VQC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    VQC_BLAS(rocblas_create_handle(&blas_));
+    VQC_BLAS(rocblas_set_stream(blas_, stream_));
+    // Deterministic library kernels: no atomics-based reductions (rocSOLVER's
+    // eigensolvers call rocBLAS through this handle too).
+    VQC_BLAS(rocblas_set_atomics_mode(blas_, rocblas_atomics_not_allowed));
+  }
   try {
     setup(auxiliary);
   } catch (...) {
-    rocblas_destroy_handle(blas_);
-    (void)hipStreamDestroy(stream_);
+    if (owns_stream_) {
+      rocblas_destroy_handle(blas_);
+      (void)hipStreamDestroy(stream_);
+    }
     throw;
   }
 }
@@ -96,6 +108,7 @@ DfPlan::DfPlan(int device, const System& orbital, const System& auxiliary, const
 DfPlan::~DfPlan() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
+  if (!owns_stream_) return;
   if (blas_) rocblas_destroy_handle(blas_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -362,6 +375,7 @@ int DfPlan::factorise_density(const double* D) {
 }
 
 JkResult DfPlan::evaluate(const std::vector<double>& density, std::vector<double>* gradient) {
+  converged_state_ = false;
   VQC_HIP(hipSetDevice(device_));
   const std::size_t nn = static_cast<std::size_t>(n_) * n_;
   if (density.size() != nn) throw std::invalid_argument("density shape mismatch");
@@ -406,6 +420,7 @@ JkResult DfPlan::evaluate(const std::vector<double>& density, std::vector<double
 }
 
 JkResult DfPlan::evaluate_occupied(const std::vector<double>& occupied, int nocc) {
+  converged_state_ = false;
   VQC_HIP(hipSetDevice(device_));
   if (nocc < 0 || occupied.size() != static_cast<std::size_t>(n_) * nocc)
     throw std::invalid_argument("occupied factor shape mismatch");
@@ -425,6 +440,7 @@ JkResult DfPlan::evaluate_occupied(const std::vector<double>& occupied, int nocc
 }
 
 std::array<double, 5> DfPlan::benchmark_occupied(const std::vector<double>& occupied, int nocc, unsigned repeat) {
+  converged_state_ = false;
   VQC_HIP(hipSetDevice(device_));
   if (repeat == 0) throw std::invalid_argument("repeat must be positive");
   if (nocc <= 0 || occupied.size() != static_cast<std::size_t>(n_) * nocc)
@@ -485,8 +501,32 @@ std::vector<double> DfPlan::overlap() const {
   return s;
 }
 
+std::vector<double> DfPlan::converged_forces() {
+  VQC_HIP(hipSetDevice(device_));
+  if (!converged_state_) throw std::logic_error("forces need a converged solve on this plan");
+  converged_state_ = false;  // the gradient overwrites U_
+  // The last build of solve() left rho_, the FP64 U_ and the factor
+  // sqrt(2) C_occ of D_ in place; eps_ holds the orbital energies of the same
+  // diagonalisation.
+  const std::size_t ng = orb_.atoms.size() * 3;
+  DeviceArray<double> g(ng);
+  VQC_HIP(hipMemsetAsync(g.data(), 0, g.bytes(), stream_));
+  one_electron_gradient(D_.data(), eps_.data(), nocc_, g.data());
+  df_two_electron_gradient(nocc_, nocc_, g.data());
+  std::vector<double> hg(ng);
+  g.download(hg.data(), ng, stream_);
+  const auto gn = nuclear_repulsion_gradient();
+  std::vector<double> forces(ng);
+  for (std::size_t k = 0; k < ng; ++k) {
+    forces[k] = -(hg[k] + gn[k]);
+    if (!std::isfinite(forces[k])) throw NumericalFailure("non-finite analytic force");
+  }
+  return forces;
+}
+
 ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial_density, bool forces) {
   VQC_HIP(hipSetDevice(device_));
+  converged_state_ = false;
   if (orb_.multiplicity != 1 || orb_.electrons % 2 != 0)
     throw NotImplemented("only closed-shell restricted SCF is implemented");
   if (!opts_.coulomb || !opts_.exchange || opts_.cj != 1.0 || opts_.ck != -0.5)
@@ -647,25 +687,13 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   sync();
   out.energy = 0.5 * (hs[0] + hs[1]) + enuc_;
   out.converged = std::isfinite(out.energy);
-  if (forces && out.converged) {
-    // Complete gradient at the converged density: the last build above left
-    // rho_, the FP64 U_ and the factor sqrt(2) C_occ of D_ in place, and w
-    // holds the orbital energies of the same diagonalisation.
-    const std::size_t ng = orb_.atoms.size() * 3;
-    DeviceArray<double> g(ng);
-    VQC_HIP(hipMemsetAsync(g.data(), 0, g.bytes(), stream_));
-    one_electron_gradient(D_.data(), w.data(), nocc, g.data());
-    df_two_electron_gradient(nocc, nocc, g.data());
-    std::vector<double> hg(ng);
-    g.download(hg.data(), ng, stream_);
+  if (out.converged) {
+    eps_.resize(static_cast<std::size_t>(n));
+    VQC_HIP(hipMemcpyAsync(eps_.data(), w.data(), n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
     sync();
-    const auto gn = nuclear_repulsion_gradient();
-    out.forces.resize(ng);
-    for (std::size_t k = 0; k < ng; ++k) {
-      out.forces[k] = -(hg[k] + gn[k]);
-      if (!std::isfinite(out.forces[k])) throw NumericalFailure("non-finite analytic force");
-    }
+    converged_state_ = true;
   }
+  if (forces && out.converged) out.forces = converged_forces();
   return out;
 }
 

@@ -44,7 +44,11 @@ struct JkResult {
 
 class DfPlan {
  public:
-  DfPlan(int device, const System& orbital, const System& auxiliary, const PlanOptions& options);
+  // With stream/blas non-null the plan runs on that stream and rocBLAS handle
+  // (owned by the caller, who keeps them alive longer than the plan);
+  // otherwise it creates its own.
+  DfPlan(int device, const System& orbital, const System& auxiliary, const PlanOptions& options,
+         hipStream_t stream = nullptr, rocblas_handle blas = nullptr);
   ~DfPlan();
   DfPlan(const DfPlan&) = delete;
   DfPlan& operator=(const DfPlan&) = delete;
@@ -77,6 +81,9 @@ class DfPlan {
 
   // With forces, the outcome carries -dE/dR of the converged energy.
   ScfOutcome solve(const ScfOptions& options, const std::vector<double>* initial_density, bool forces = false);
+  // -dE/dR (3 natom) at the density the last successful solve() converged to;
+  // valid until the next operation on this plan.
+  std::vector<double> converged_forces();
 
  private:
   void setup(const System& aux);
@@ -99,6 +106,7 @@ class DfPlan {
   int device_;
   hipStream_t stream_ = nullptr;
   rocblas_handle blas_ = nullptr;
+  bool owns_stream_ = true;
   System orb_, aux_;
   PlanOptions opts_;
   Precision prec_ = Precision::FP64;
@@ -113,6 +121,8 @@ class DfPlan {
   DeviceBasis ob_;        // orbital basis as the integral kernels see it
   AuxiliaryLayout al_;    // auxiliary functions in internal order
   DeviceArray<double> L_;  // Cholesky factor of (P|Q), column-major lower (Cholesky path only)
+  DeviceArray<double> eps_;  // orbital energies of the last converged solve
+  bool converged_state_ = false;  // D_, c64_, rho_, FP64 U_ and eps_ describe that solve
   DeviceArray<double> S_, H_;
   DeviceArray<double> B64_;
   DeviceArray<float> B32_;

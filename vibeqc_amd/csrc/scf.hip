@@ -88,55 +88,69 @@ __global__ __launch_bounds__(kReduceThreads) void diis_gram_kernel(const double*
 }
 
 // Augmented Pulay system [G -1; -1 0][c; l] = [0; -1] by Gaussian elimination
-// with partial pivoting, one thread (m <= 32).
-__global__ void diis_solve_kernel(const double* __restrict__ gram, int m, const int* __restrict__ slot_of, int cap,
-                                  double* __restrict__ coef) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// with partial pivoting (m <= 32), one wavefront: the system lives in LDS,
+// lanes own rows for the elimination and columns for the swaps, and lane 0
+// picks pivots and back-substitutes, so the result is deterministic and the
+// kernel needs no private (scratch) memory.
+__global__ __launch_bounds__(64) void diis_solve_kernel(const double* __restrict__ gram, int m,
+                                                        const int* __restrict__ slot_of, int cap,
+                                                        double* __restrict__ coef) {
   constexpr int kMax = 33;
-  double a[kMax][kMax + 1];
+  __shared__ double a[kMax][kMax + 1];
+  __shared__ double x[kMax];
+  __shared__ int piv_row, ok_flag;
+  const int lane = threadIdx.x;
   const int d = m + 1;
-  for (int i = 0; i < m; ++i) {
-    for (int j = 0; j < m; ++j) a[i][j] = gram[slot_of[i] * cap + slot_of[j]];
-    a[i][m] = -1.0;
-    a[i][d] = 0.0;
+  for (int e = lane; e < d * (d + 1); e += 64) {
+    const int i = e / (d + 1), j = e % (d + 1);
+    double v;
+    if (i < m && j < m) v = gram[slot_of[i] * cap + slot_of[j]];
+    else if (i < m) v = j == m ? -1.0 : 0.0;       // column m: -1, rhs: 0
+    else v = j < m ? -1.0 : (j == m ? 0.0 : -1.0);  // last row: -1 ... 0 | -1
+    a[i][j] = v;
   }
-  for (int j = 0; j < m; ++j) a[m][j] = -1.0;
-  a[m][m] = 0.0;
-  a[m][d] = -1.0;
-  bool ok = true;
+  if (lane == 0) ok_flag = 1;
+  __syncthreads();
   double scale = 0.0;
-  for (int i = 0; i < d; ++i)
-    for (int j = 0; j < d; ++j) scale = fmax(scale, fabs(a[i][j]));
-  for (int c = 0; c < d && ok; ++c) {
-    int piv = c;
-    for (int r = c + 1; r < d; ++r)
-      if (fabs(a[r][c]) > fabs(a[piv][c])) piv = r;
-    if (!(fabs(a[piv][c]) > 1e-300) || !(fabs(a[piv][c]) > scale * 1e-300)) {
-      ok = false;
-      break;
+  if (lane == 0)
+    for (int i = 0; i < d; ++i)
+      for (int j = 0; j < d; ++j) scale = fmax(scale, fabs(a[i][j]));
+  for (int c = 0; c < d; ++c) {
+    if (lane == 0) {
+      int p = c;
+      for (int r = c + 1; r < d; ++r)
+        if (fabs(a[r][c]) > fabs(a[p][c])) p = r;
+      piv_row = p;
+      if (!(fabs(a[p][c]) > 1e-300) || !(fabs(a[p][c]) > scale * 1e-300)) ok_flag = 0;
     }
-    if (piv != c)
-      for (int j = 0; j <= d; ++j) {
-        const double t = a[c][j];
-        a[c][j] = a[piv][j];
-        a[piv][j] = t;
+    __syncthreads();
+    if (!ok_flag) break;
+    const int p = piv_row;
+    if (p != c && lane <= d) {
+      const double t = a[c][lane];
+      a[c][lane] = a[p][lane];
+      a[p][lane] = t;
+    }
+    __syncthreads();
+    if (lane > c && lane < d) {
+      const double f = a[lane][c] / a[c][c];
+      for (int j = c; j <= d; ++j) a[lane][j] -= f * a[c][j];
+    }
+    __syncthreads();
+  }
+  if (lane == 0) {
+    bool ok = ok_flag != 0;
+    if (ok) {
+      for (int i = d - 1; i >= 0; --i) {
+        double t = a[i][d];
+        for (int j = i + 1; j < d; ++j) t -= a[i][j] * x[j];
+        x[i] = t / a[i][i];
       }
-    for (int r = c + 1; r < d; ++r) {
-      const double f = a[r][c] / a[c][c];
-      for (int j = c; j <= d; ++j) a[r][j] -= f * a[c][j];
+      for (int i = 0; i < m; ++i)
+        if (!isfinite(x[i])) ok = false;
     }
+    for (int i = 0; i < m; ++i) coef[i] = ok ? x[i] : (i == m - 1 ? 1.0 : 0.0);
   }
-  double x[kMax];
-  if (ok) {
-    for (int i = d - 1; i >= 0; --i) {
-      double s = a[i][d];
-      for (int j = i + 1; j < d; ++j) s -= a[i][j] * x[j];
-      x[i] = s / a[i][i];
-    }
-    for (int i = 0; i < m; ++i)
-      if (!isfinite(x[i])) ok = false;
-  }
-  for (int i = 0; i < m; ++i) coef[i] = ok ? x[i] : (i == m - 1 ? 1.0 : 0.0);
 }
 
 __global__ void diis_extrapolate_kernel(const double* __restrict__ focks, std::size_t count, int m,

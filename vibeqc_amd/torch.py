@@ -11,6 +11,7 @@ derivatives are refused as in the reference (torch.py:34-44).
 
 from __future__ import annotations
 
+import numpy as np
 import torch
 
 from .calculator import Calculator
@@ -83,11 +84,43 @@ def energy(coordinates, atomic_numbers, calculator: Calculator | None = None, *,
     return _EnergyFunction.apply(coordinates, z, calculator, int(charge), int(multiplicity))
 
 
-def batched_energy(coordinates, atomic_numbers, calculator: Calculator | None = None, charges=None, multiplicities=None):
-    """Energies of several molecules (one tensor per molecule), stacked."""
+def batched_energy(coordinates, atomic_numbers, calculator: Calculator | None = None, charges=None, multiplicities=None,
+                   prepared_batch=None):
+    """Energies of several molecules (one (natoms, 3) tensor each), stacked;
+    the molecules run as one ragged batch (torch.py:187-245).  Backward uses
+    the batch's analytic forces."""
     calculator = calculator or Calculator()
-    charges = charges or [0] * len(coordinates)
-    multiplicities = multiplicities or [1] * len(coordinates)
-    return torch.stack(
-        [energy(c, z, calculator, charge=q, multiplicity=m) for c, z, q, m in zip(coordinates, atomic_numbers, charges, multiplicities)]
-    )
+    coords = list(coordinates)
+    zs = [_validated(c, z) for c, z in zip(coords, atomic_numbers)]
+    if len(zs) != len(coords):
+        raise ValueError("one atomic-number list per coordinate tensor is required")
+    return _BatchedEnergyFunction.apply(calculator, zs, charges, multiplicities, prepared_batch, *coords)
+
+
+class _BatchedEnergyFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, calculator, zs, charges, multiplicities, prepared_batch, *coords):
+        systems = [
+            [(z[i], c.detach().cpu().double()[i].tolist()) for i in range(c.shape[0])] for c, z in zip(coords, zs)
+        ]
+        want = ("energy", "forces") if any(c.requires_grad for c in coords) else ("energy",)
+        if prepared_batch is None:
+            result = calculator.batch_singlepoint(
+                systems, charges=charges, multiplicities=multiplicities, strict=True, properties=want
+            )
+        else:
+            xyz = [np.asarray([p for _, p in s]) for s in systems]
+            result = prepared_batch.execute(xyz, strict=True, properties=want)
+        forces = [
+            torch.as_tensor(it.forces, dtype=c.dtype, device=c.device) if it.forces is not None else None
+            for it, c in zip(result.items, coords)
+        ]
+        ctx.forces = forces
+        ref = coords[0]
+        return torch.as_tensor(result.energies, dtype=ref.dtype, device=ref.device)
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        _require_first_order_backward()
+        grads = tuple(-f * grad_output[i] if f is not None else None for i, f in enumerate(ctx.forces))
+        return (None, None, None, None, None, *grads)
