@@ -11,7 +11,6 @@
 #include <limits>
 #include <map>
 #include <mutex>
-#include <shared_mutex>
 #include <stdexcept>
 #include <thread>
 #include <tuple>
@@ -19,14 +18,35 @@
 namespace vqc {
 namespace {
 
-// Keep freed stream-ordered memory in the device pool for the next plan
-// rather than returning it to the driver at every synchronisation.
-void retain_pool(int device) {
+// One stream-ordered memory pool per worker, kept mapped between plans
+// (release threshold: never) and released as a whole when destroyed.
+hipMemPool_t worker_pool(int device) {
+  hipMemPoolProps props{};
+  props.allocType = hipMemAllocationTypePinned;
+  props.location.type = hipMemLocationTypeDevice;
+  props.location.id = device;
   hipMemPool_t pool = nullptr;
-  if (hipDeviceGetDefaultMemPool(&pool, device) != hipSuccess || !pool) return;
+  VQC_HIP(hipMemPoolCreate(&pool, &props));
   uint64_t threshold = std::numeric_limits<uint64_t>::max();
   (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+  return pool;
 }
+
+// Routes this thread's device allocations to a worker's stream and pool for
+// the lifetime of the object.
+class AllocationScope {
+ public:
+  AllocationScope(hipStream_t s, hipMemPool_t p) {
+    allocation_stream() = s;
+    allocation_pool() = p;
+  }
+  ~AllocationScope() {
+    allocation_stream() = nullptr;
+    allocation_pool() = nullptr;
+  }
+  AllocationScope(const AllocationScope&) = delete;
+  AllocationScope& operator=(const AllocationScope&) = delete;
+};
 
 void move_atoms(System& sys, const std::vector<double>& xyz) {
   if (xyz.size() != sys.atoms.size() * 3) throw std::invalid_argument("batch coordinates do not match the atom count");
@@ -75,33 +95,28 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
     throw std::invalid_argument("batch coordinate count mismatch");
   if (forces.size() != count) throw std::invalid_argument("batch force request count mismatch");
   VQC_HIP(hipSetDevice(device_));
-  retain_pool(device_);
   std::vector<BatchItemOutcome> out(count);
-  std::atomic<std::size_t> next{0};
   // Worker count: the option, else VIBEQC_BATCH_WORKERS, else 8.
   int requested = opts_.workers;
   if (requested <= 0)
     if (const char* env = std::getenv("VIBEQC_BATCH_WORKERS")) requested = std::atoi(env);
   const int workers = std::max(1, std::min<int>(requested > 0 ? requested : 8, static_cast<int>(count)));
-  // Plan setup (integral generation, metric factorisation, B) runs alone on
-  // the device: while a setup is in flight no other worker's kernels run, and
-  // SCF phases of different systems overlap freely.  Setups overlapping other
-  // work were observed to perturb results at the 1e-8..1e-4 Eh level; with
-  // setups exclusive, repeated executions are bitwise reproducible.
-  std::shared_mutex phase_mutex;
   // One stream and rocBLAS handle per worker, created before and destroyed
   // after all plans that borrow them.
   std::vector<hipStream_t> streams(workers, nullptr);
   std::vector<rocblas_handle> handles(workers, nullptr);
+  std::vector<hipMemPool_t> pools(workers, nullptr);
   auto release_resources = [&]() {
     for (int w = 0; w < workers; ++w) {
       if (streams[w]) (void)hipStreamSynchronize(streams[w]);
       if (handles[w]) rocblas_destroy_handle(handles[w]);
       if (streams[w]) (void)hipStreamDestroy(streams[w]);
+      if (pools[w]) (void)hipMemPoolDestroy(pools[w]);
     }
   };
   try {
     for (int w = 0; w < workers; ++w) {
+      pools[w] = worker_pool(device_);
       VQC_HIP(hipStreamCreateWithFlags(&streams[w], hipStreamNonBlocking));
       VQC_BLAS(rocblas_create_handle(&handles[w]));
       VQC_BLAS(rocblas_set_stream(handles[w], streams[w]));
@@ -112,65 +127,96 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
     release_resources();
     throw;
   }
-  auto work = [&](int w) {
-    const hipStream_t stream = streams[w];
-    const rocblas_handle blas = handles[w];
-    if (hipSetDevice(device_) != hipSuccess) return;
-    allocation_stream() = stream;
-    for (std::size_t k = next.fetch_add(1); k < count; k = next.fetch_add(1)) {
-      const std::size_t i = order_[k];
-      BatchItemOutcome& r = out[i];
-      r.bucket = bucket_[i];
-      try {
-        System sys = systems_[i];
-        System aux = aux_[i];
-        if (!coordinates.empty() && coordinates[i]) {
-          move_atoms(sys, *coordinates[i]);
-          move_atoms(aux, *coordinates[i]);
-        }
-        std::unique_ptr<DfPlan> owned;
-        {
-          std::unique_lock<std::shared_mutex> setup(phase_mutex);
-          owned = std::make_unique<DfPlan>(device_, sys, aux, opts_.plan, stream, blas);
-          VQC_HIP(hipStreamSynchronize(stream));
-        }
-        std::shared_lock<std::shared_mutex> running(phase_mutex);
-        DfPlan& plan = *owned;
-        plan.set_precision(opts_.precision);
-        const std::vector<double>* seed = nullptr;
-        if (opts_.warm_starts && warm_[i].size() == sys.nbf() * sys.nbf()) {
-          seed = &warm_[i];
-          r.warm_start_used = true;
-        }
-        try {
-          r.scf = plan.solve(opts_.scf, seed, false);
-        } catch (const std::invalid_argument&) {
-          if (!seed) throw;
-          // A retained density the guards reject falls back to the core guess.
-          r.warm_start_used = false;
-          r.warm_start_fallback = true;
-          r.scf = plan.solve(opts_.scf, nullptr, false);
-        }
-        running.unlock();
-        if (forces[i] && r.scf.converged) {
-          // Derivative integrals, like setup, run with the device to themselves.
-          std::unique_lock<std::shared_mutex> exclusive(phase_mutex);
-          r.scf.forces = plan.converged_forces();
-          VQC_HIP(hipStreamSynchronize(stream));
-        }
-        if (r.scf.converged && opts_.warm_starts && warm_updates_) warm_[i] = r.scf.density;
-      } catch (...) {
-        r.error = std::current_exception();
+  // Execution in waves of the largest-first order.  Within a wave the plans
+  // are set up one after another (integral generation, metric factorisation,
+  // B), then their SCFs run concurrently, one worker per stream, and forces
+  // are evaluated one system at a time.  Setup and derivative-integral
+  // kernels therefore never share the device with another system's work: when
+  // they did, results were perturbed at the 1e-8..1e-4 Eh level; with the
+  // phases separated, repeated executions are bitwise reproducible.
+  const std::size_t wave = static_cast<std::size_t>(workers) * 4;
+  std::vector<std::unique_ptr<DfPlan>> plans(count);
+  auto setup_item = [&](std::size_t k) {
+    const std::size_t i = order_[k];
+    const int w = static_cast<int>(k % workers);
+    BatchItemOutcome& r = out[i];
+    r.bucket = bucket_[i];
+    AllocationScope scope(streams[w], pools[w]);
+    try {
+      System sys = systems_[i];
+      System aux = aux_[i];
+      if (!coordinates.empty() && coordinates[i]) {
+        move_atoms(sys, *coordinates[i]);
+        move_atoms(aux, *coordinates[i]);
       }
+      plans[k] = std::make_unique<DfPlan>(device_, sys, aux, opts_.plan, streams[w], handles[w]);
+      plans[k]->set_precision(opts_.precision);
+      VQC_HIP(hipStreamSynchronize(streams[w]));
+    } catch (...) {
+      r.error = std::current_exception();
+      plans[k].reset();
     }
-    allocation_stream() = nullptr;
-    (void)hipStreamSynchronize(stream);
   };
-  std::vector<std::thread> pool;
-  pool.reserve(workers - 1);
-  for (int w = 1; w < workers; ++w) pool.emplace_back(work, w);
-  work(0);
-  for (auto& t : pool) t.join();
+  auto solve_item = [&](std::size_t k) {
+    const std::size_t i = order_[k];
+    BatchItemOutcome& r = out[i];
+    if (!plans[k]) return;
+    try {
+      DfPlan& plan = *plans[k];
+      const std::vector<double>* seed = nullptr;
+      if (opts_.warm_starts && warm_[i].size() == static_cast<std::size_t>(plan.nbf()) * plan.nbf()) {
+        seed = &warm_[i];
+        r.warm_start_used = true;
+      }
+      try {
+        r.scf = plan.solve(opts_.scf, seed, false);
+      } catch (const std::invalid_argument&) {
+        if (!seed) throw;
+        // A retained density the guards reject falls back to the core guess.
+        r.warm_start_used = false;
+        r.warm_start_fallback = true;
+        r.scf = plan.solve(opts_.scf, nullptr, false);
+      }
+    } catch (...) {
+      r.error = std::current_exception();
+    }
+  };
+  try {
+    for (std::size_t w0 = 0; w0 < count; w0 += wave) {
+      const std::size_t w1 = std::min(count, w0 + wave);
+      for (std::size_t k = w0; k < w1; ++k) setup_item(k);
+      auto run = [&](int w) {
+        if (hipSetDevice(device_) != hipSuccess) return;
+        AllocationScope scope(streams[w], pools[w]);
+        for (std::size_t k = w0 + static_cast<std::size_t>(w); k < w1; k += workers) solve_item(k);
+        (void)hipStreamSynchronize(streams[w]);
+      };
+      std::vector<std::thread> pool;
+      for (int w = 1; w < workers; ++w) pool.emplace_back(run, w);
+      run(0);
+      for (auto& t : pool) t.join();
+      for (std::size_t k = w0; k < w1; ++k) {
+        const std::size_t i = order_[k];
+        BatchItemOutcome& r = out[i];
+        if (!plans[k] || r.error || !r.scf.converged) continue;
+        try {
+          if (forces[i]) {
+            AllocationScope scope(streams[k % workers], pools[k % workers]);
+            r.scf.forces = plans[k]->converged_forces();
+          }
+          if (opts_.warm_starts && warm_updates_) warm_[i] = r.scf.density;
+        } catch (...) {
+          r.error = std::current_exception();
+        }
+      }
+      // Plans free into the pool of the worker that allocated them.
+      for (std::size_t k = w0; k < w1; ++k) plans[k].reset();
+    }
+  } catch (...) {
+    plans.clear();
+    release_resources();
+    throw;
+  }
   release_resources();
   return out;
 }

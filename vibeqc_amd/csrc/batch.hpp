@@ -3,12 +3,11 @@
 // (reference generativeqc.h:1431-1586, src/scf/fleet.cpp).
 //
 // Systems are grouped into shape buckets (nbf, naux, nocc), as the reference
-// groups its fleets (fleet.cpp:26-36), and executed largest first by a pool of
-// host workers.  Each worker owns one HIP stream and one rocBLAS handle and
-// allocates stream-ordered, so the many small plans of a ragged batch run
-// concurrently on the GPU instead of one after another.  A plan lives only
-// while its system is solved, which bounds device memory by the largest few
-// systems rather than the whole batch.
+// groups its fleets (fleet.cpp:26-36), and executed largest first in waves of
+// 4 x workers systems.  Each host worker owns one HIP stream and one rocBLAS
+// handle and allocates stream-ordered, so the SCFs of a wave run concurrently
+// on the GPU instead of one after another.  A plan lives only for its wave,
+// which bounds device memory by the largest wave rather than the whole batch.
 #pragma once
 
 #include <exception>

@@ -19,6 +19,7 @@
 
 #include "integrals_gradient.hpp"
 #include "plan.hpp"
+#include "scf.hpp"
 
 namespace vqc {
 namespace {
@@ -84,7 +85,7 @@ void DfPlan::df_two_electron_gradient(int r, int r_pos, double* grad) {
   const double one = 1.0, zero = 0.0;
   // c = L^-T rho_B (padding rows stay zero).
   DeviceArray<double> c(lq);
-  VQC_HIP(hipMemcpyAsync(c.data(), rho_.data(), lq * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  copy_device(c.data(), rho_.data(), lq, stream_);
   VQC_BLAS(rocblas_dtrsm(blas_, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
                          rocblas_diagonal_non_unit, na, 1, &one, L_.data(), na, c.data(), lq));
   DeviceArray<double> WM(static_cast<std::size_t>(na) * na);

@@ -180,7 +180,7 @@ void DfPlan::factor_metric(DeviceArray<double>& M, DeviceArray<double>& A) {
   extract_diagonal(M.data(), na, diag.data(), stream_);
   diag.download(mdiag.data(), na, stream_);
   DeviceArray<double> L(mm);
-  VQC_HIP(hipMemcpyAsync(L.data(), M.data(), mm * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  copy_device(L.data(), M.data(), mm, stream_);
   DeviceArray<rocblas_int> info(1);
   VQC_BLAS(rocsolver_dpotrf(blas_, rocblas_fill_lower, na, L.data(), na, info.data()));
   rocblas_int hinfo = 0;
@@ -213,7 +213,7 @@ void DfPlan::factor_metric(DeviceArray<double>& M, DeviceArray<double>& A) {
   // Eigen-cutoff route: X = V diag(lambda^-1/2 | lambda > thr lambda_max) V^T, B = A X.
   cholesky_ = false;
   DeviceArray<double> V(mm), w(na), e(na);
-  VQC_HIP(hipMemcpyAsync(V.data(), M.data(), mm * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  copy_device(V.data(), M.data(), mm, stream_);
   VQC_BLAS(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_lower, na, V.data(), na, w.data(), e.data(),
                             info.data()));
   std::vector<double> hw(na);
@@ -249,8 +249,7 @@ void DfPlan::factor_metric(DeviceArray<double>& M, DeviceArray<double>& A) {
     const int cols = static_cast<int>(std::min(kTrsmColumns, npair_ - c0));
     VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, na, cols, na, &one, X.data(), na,
                            A.data() + c0 * ldq_, ldq_, &zero, tmp.data(), ldq_));
-    VQC_HIP(hipMemcpyAsync(A.data() + c0 * ldq_, tmp.data(), static_cast<std::size_t>(cols) * ldq_ * sizeof(double),
-                           hipMemcpyDeviceToDevice, stream_));
+    copy_device(A.data() + c0 * ldq_, tmp.data(), static_cast<std::size_t>(cols) * ldq_, stream_);
   }
   sync();
 }
@@ -347,7 +346,7 @@ int DfPlan::factorise_density(const double* D) {
   const std::size_t nn = static_cast<std::size_t>(n_) * n_;
   DeviceArray<double> V(nn), w(n_), e(n_);
   DeviceArray<rocblas_int> info(1);
-  VQC_HIP(hipMemcpyAsync(V.data(), D, nn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  copy_device(V.data(), D, nn, stream_);
   VQC_BLAS(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_lower, n_, V.data(), n_, w.data(), e.data(),
                             info.data()));
   std::vector<double> hw(n_);
@@ -541,7 +540,7 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   DeviceArray<rocblas_int> info(1);
   DeviceArray<double> scalars(4);
   auto eigh = [&](const double* A, double* vectors, double* values) {
-    VQC_HIP(hipMemcpyAsync(vectors, A, nn * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    copy_device(vectors, A, nn, stream_);
     VQC_BLAS(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_lower, n, vectors, n, values, e.data(),
                               info.data()));
   };
@@ -559,15 +558,77 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   }
   scale_columns_inv_sqrt(V.data(), w.data(), n, W.data(), stream_);
   gemm(rocblas_operation_none, rocblas_operation_transpose, n, W.data(), n, V.data(), n, X.data());
-  // Orbitals of F in the X basis: C = X eig(X F X); new factor and density.
-  auto new_density = [&](const double* Fm, double* Dout) {
+  // New density from F.  The occupied space of F~ = X F X comes from a
+  // diagonalisation (first iteration, after a rejected purification and for
+  // the final canonical orbitals) or, otherwise, from SP2 purification of F~
+  // followed by one subspace step from the previous occupied orbitals Cx:
+  // Q = P Cx, Cx <- Q chol(Q^T Q)^-T, which spans the same space as the
+  // lowest nocc eigenvectors and is orthonormal.  The purification is GEMMs
+  // and elementwise kernels, where the eigensolver is a long chain of small
+  // launches (its tridiagonal reduction alone is most of an SCF iteration
+  // for a few hundred functions).
+  const bool sp2_usable = nocc > 0 && nocc < n;
+  const std::size_t nx = sp2_usable ? static_cast<std::size_t>(n) * nocc : 1;
+  DeviceArray<double> Cx(nx), Qb(nx), Mb(sp2_usable ? static_cast<std::size_t>(nocc) * nocc : 1),
+      sp2_state(kSp2StateSize), sp2_partial(sp2_partial_size(n));
+  DeviceArray<int> sp2_status(1);
+  DeviceArray<rocblas_int> minfo(1);
+  bool have_cx = false, check_sp2 = false;
+  int sp2_steps = 0;
+  double sp2_lo = 0.0, sp2_hi = 0.0;
+  const double dnocc = static_cast<double>(nocc);
+  // Purify T2 into V; false when it does not reach an integer trace.
+  auto purify = [&]() {
+    sp2_begin(T2.data(), n, sp2_partial.data(), sp2_state.data(), V.data(), stream_);
+    int steps = 0, batch = sp2_hint_ > 0 ? sp2_hint_ : 24;
+    double hstate[8];
+    for (;;) {
+      for (int j = 0; j < batch && steps < kSp2MaxSteps; ++j, ++steps) {
+        gemm(rocblas_operation_none, rocblas_operation_none, n, V.data(), n, V.data(), n, W.data());
+        sp2_step(V.data(), W.data(), n, dnocc, sp2_state.data(), steps, stream_);
+      }
+      sp2_state.download(hstate, 8, stream_);
+      const Sp2Summary sum = sp2_summary(hstate, steps);
+      sp2_lo = hstate[4];
+      sp2_hi = hstate[5];
+      if (sum.converged && std::abs(sum.trace - dnocc) < 1e-6) {
+        sp2_hint_ = sum.applied;
+        sp2_steps = steps;
+        return true;
+      }
+      if (sum.converged || sum.degenerate || steps >= kSp2MaxSteps) return false;
+      batch = 4;
+    }
+  };
+  auto new_density = [&](const double* Fm, double* Dout, bool canonical) {
     gemm(rocblas_operation_none, rocblas_operation_none, n, Fm, n, X.data(), n, T1.data());
     gemm(rocblas_operation_none, rocblas_operation_none, n, X.data(), n, T1.data(), n, T2.data());
-    eigh(T2.data(), V.data(), w.data());
-    gemm(rocblas_operation_none, rocblas_operation_none, n, X.data(), n, V.data(), n, W.data());
+    check_sp2 = false;
+    if (!canonical && have_cx && purify()) {
+      VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, n, nocc, n, &one, V.data(), n,
+                             Cx.data(), n, &zero, Qb.data(), n));
+      VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, nocc, nocc, n, &one,
+                             Qb.data(), n, Qb.data(), n, &zero, Mb.data(), nocc));
+      VQC_BLAS(rocsolver_dpotrf(blas_, rocblas_fill_lower, nocc, Mb.data(), nocc, minfo.data()));
+      VQC_BLAS(rocblas_dtrsm(blas_, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                             rocblas_diagonal_non_unit, n, nocc, &one, Mb.data(), nocc, Qb.data(), n));
+      sp2_validate(Mb.data(), nocc, minfo.data(), sp2_state.data(), sp2_steps, dnocc, sp2_status.data(), stream_);
+      std::swap(Cx, Qb);
+      VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, n, nocc, n, &one, X.data(), n,
+                             Cx.data(), n, &zero, W.data(), n));
+      check_sp2 = true;
+    } else {
+      eigh(T2.data(), V.data(), w.data());
+      gemm(rocblas_operation_none, rocblas_operation_none, n, X.data(), n, V.data(), n, W.data());
+      if (sp2_usable) {
+        copy_device(Cx.data(), V.data(), nx, stream_);
+        have_cx = true;
+      }
+    }
     occupied_from_orbitals(W.data(), n, nocc, ldc_, std::sqrt(2.0), c64_.data(), stream_);
     VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, n, n, std::max(nocc, 1), &one,
                            c64_.data(), ldc_, c64_.data(), ldc_, &zero, Dout, n));
+    return check_sp2;
   };
   ScfOutcome out;
   // Initial density: core Hamiltonian guess (src/scf/initial_guess/density.cpp:635-672) or caller seed.
@@ -577,7 +638,7 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
     const int packed = factorise_density(D_.data());
     if ((packed >> 16) != (packed & 0xffff)) throw std::invalid_argument("initial density must be positive semidefinite");
   } else {
-    new_density(H_.data(), D_.data());
+    new_density(H_.data(), D_.data(), true);
   }
   int rank_now = initial_density ? -1 : nocc;
   auto current_rank = [&]() { return rank_now; };
@@ -589,13 +650,23 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   int next_slot = 0;
 
   const bool reduced = prec_ != Precision::FP64;
-  const double e_switch = prec_ == Precision::BF16X3 ? 1e-6 : 1e-8;
-  const double d_switch = prec_ == Precision::BF16X3 ? 1e-5 : 1e-7;
+  // VIBEQC_TRACE=scf: one JSON line per iteration on stderr.
+  const char* trace_env = std::getenv("VIBEQC_TRACE");
+  const bool trace_scf = trace_env && std::strcmp(trace_env, "scf") == 0;
+  // Hand-over to FP64: BF16X3 Fock matrices carry ~16 significant bits and
+  // its stage stops at 1e-4 Eh / 1e-3 RMS density change (FP32: 1e-6 /
+  // 1e-5), or when the density change has not halved its best for three
+  // iterations.  Later hand-overs cost more reduced iterations than the FP64
+  // refinement saves (measured on water clusters of 1-16 monomers).
+  const double e_switch = prec_ == Precision::BF16X3 ? 1e-4 : 1e-6;
+  const double d_switch = prec_ == Precision::BF16X3 ? 1e-3 : 1e-5;
+  double best_rms = std::numeric_limits<double>::infinity();
+  int stalled = 0;
   bool in_reduced = reduced;
   double previous = std::numeric_limits<double>::infinity();
   bool converged = false;
   unsigned it = 0;
-  double energy = 0.0, de = 0.0, rms = 0.0;
+  double energy = 0.0, de = 0.0, rms = 0.0, last_rms = std::numeric_limits<double>::infinity();
   for (it = 1; it <= o.max_iterations; ++it) {
     const Precision p = in_reduced ? prec_ : Precision::FP64;
     int r = nocc, rp = nocc;
@@ -624,8 +695,7 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
         order.erase(order.begin());
       }
       order.push_back(slot);
-      VQC_HIP(hipMemcpyAsync(focks.data() + static_cast<std::size_t>(slot) * nn, F.data(), nn * sizeof(double),
-                             hipMemcpyDeviceToDevice, stream_));
+      copy_device(focks.data() + static_cast<std::size_t>(slot) * nn, F.data(), nn, stream_);
       antisymmetrise(T2.data(), n, errors.data() + static_cast<std::size_t>(slot) * nn, stream_);
       const int m = static_cast<int>(order.size());
       slots.upload(order.data(), m, stream_);
@@ -636,21 +706,49 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
         Fuse = Fe.data();
       }
     }
-    new_density(Fuse, Dn.data());
+    // Purify once the density has settled (RMS change below 1e-2): earlier,
+    // the occupied space turns too far between iterations for one subspace
+    // step and the iteration would diagonalise after all.
+    const bool purified = new_density(Fuse, Dn.data(), !(last_rms < 1e-2));
     rank_now = nocc;
     rms_difference(Dn.data(), D_.data(), nn, scalars.data() + 2, stream_);
     double hs[4];
     scalars.download(hs, 4, stream_);
+    if (purified) {
+      int status = 1;
+      sp2_status.download(&status, 1, stream_);
+      if (status != 0) {
+        // Rejected subspace step (rotated too far, or a near-degenerate
+        // Fermi level): this iteration diagonalises instead.
+        new_density(Fuse, Dn.data(), true);
+        rms_difference(Dn.data(), D_.data(), nn, scalars.data() + 2, stream_);
+        scalars.download(hs, 4, stream_);
+      } else {
+        ++out.purified_iterations;
+      }
+    }
     sync();
     energy = 0.5 * (hs[0] + hs[1]) + enuc_;
     rms = hs[2];
     if (!std::isfinite(energy) || !std::isfinite(rms)) throw NumericalFailure("non-finite SCF iterate");
+    last_rms = rms;
+    if (trace_scf)
+      std::fprintf(stderr,
+                   "{\"trace\": \"scf\", \"enuc\": %.17g, \"it\": %u, \"e1\": %.17g, \"e2\": %.17g, \"rms\": %.17g, "
+                   "\"purified\": %d, \"sp2_steps\": %d, \"lo\": %.17g, \"hi\": %.17g}\n",
+                   enuc_, it, hs[0], hs[1], rms, purified ? 1 : 0, purified ? sp2_steps : 0, sp2_lo, sp2_hi);
     de = std::isfinite(previous) ? std::abs(energy - previous) : std::numeric_limits<double>::infinity();
     std::swap(D_, Dn);
     if (in_reduced) {
       const bool settled = it > 1 && de < std::max(e_switch, o.energy_tolerance) &&
                            rms < std::max(d_switch, o.density_tolerance);
-      if (settled || it >= o.max_iterations / 2) {
+      if (rms < 0.5 * best_rms) {
+        best_rms = rms;
+        stalled = 0;
+      } else {
+        ++stalled;
+      }
+      if (settled || stalled >= 3 || it >= o.max_iterations / 2) {
         // Refinement: continue in FP64 with a fresh DIIS history.
         in_reduced = false;
         order.clear();
@@ -675,7 +773,7 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   // Finalisation (mean_field_driver.cpp:181-199): rebuild, diagonalise, rebuild.
   build_jk(Precision::FP64, D_.data(), nocc, nocc);
   assemble_fock(H_.data(), J_.data(), K_.data(), opts_.cj, opts_.ck, nn, F.data(), stream_);
-  new_density(F.data(), D_.data());
+  new_density(F.data(), D_.data(), true);
   build_jk(Precision::FP64, D_.data(), nocc, nocc);
   assemble_fock(H_.data(), J_.data(), K_.data(), opts_.cj, opts_.ck, nn, F.data(), stream_);
   dot_pair(D_.data(), H_.data(), F.data(), nn, scalars.data(), stream_);
@@ -689,7 +787,7 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   out.converged = std::isfinite(out.energy);
   if (out.converged) {
     eps_.resize(static_cast<std::size_t>(n));
-    VQC_HIP(hipMemcpyAsync(eps_.data(), w.data(), n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    copy_device(eps_.data(), w.data(), n, stream_);
     sync();
     converged_state_ = true;
   }

@@ -33,6 +33,7 @@ struct ScfOutcome {
   double energy = 0.0, energy_change = 0.0, density_rms = 0.0;
   unsigned iterations = 0;
   unsigned reduced_iterations = 0;  // iterations run at reduced precision
+  unsigned purified_iterations = 0;  // new densities from SP2 purification (the rest diagonalise)
   uint64_t fock_builds = 0;
   std::vector<double> density;
   std::vector<double> forces;  // 3 natom, -dE/dR, when requested
@@ -110,6 +111,7 @@ class DfPlan {
   System orb_, aux_;
   PlanOptions opts_;
   Precision prec_ = Precision::FP64;
+  int sp2_hint_ = 0;  // SP2 steps of the last purification: the next one's first batch
   int n_ = 0, naux_ = 0, ldq_ = 0, nocc_ = 0;
   std::size_t npair_ = 0;
   int rank_ = 0;

@@ -206,11 +206,171 @@ __global__ void diagonal_kernel(const double* __restrict__ A, int n, double* __r
   if (i < n) diag[i] = A[static_cast<std::size_t>(i) * n + i];
 }
 
+
+// ---- SP2 purification ------------------------------------------------------
+// State layout (doubles): [0..1] trace of X before step k in slot k & 1,
+// [2..3] freeze flag in slot 2 + (k & 1), [4] lo, [5] hi, [6] applied
+// steps, [7] degenerate interval, [8 + k] idempotency error of step k.
+// Every block of a step reads the slots of k and block 0 writes those of
+// k + 1, so no block reads what another block of the same launch writes.
+constexpr int kSp2Cols = 64, kSp2Lanes = 4, kSp2Rows = 256;
+
+// partial[y * n + c] = sum over rows r of chunk y, r != c, of |F[r][c]|.
+__global__ __launch_bounds__(kSp2Cols* kSp2Lanes) void sp2_radius_kernel(const double* __restrict__ F, int n,
+                                                                         double* __restrict__ partial) {
+  __shared__ double sh[kSp2Lanes][kSp2Cols];
+  const int cl = threadIdx.x % kSp2Cols, lane = threadIdx.x / kSp2Cols;
+  const int c = blockIdx.x * kSp2Cols + cl;
+  const int r0 = blockIdx.y * kSp2Rows, r1 = min(n, r0 + kSp2Rows);
+  double a = 0.0;
+  if (c < n)
+    for (int r = r0 + lane; r < r1; r += kSp2Lanes)
+      if (r != c) a += fabs(F[static_cast<std::size_t>(r) * n + c]);
+  sh[lane][cl] = a;
+  __syncthreads();
+  if (lane == 0 && c < n) {
+    double t = sh[0][cl];
+    for (int l = 1; l < kSp2Lanes; ++l) t += sh[l][cl];
+    partial[static_cast<std::size_t>(blockIdx.y) * n + c] = t;
+  }
+}
+
+// Gershgorin interval [lo, hi] of F and tr X0 into the state.
+__global__ __launch_bounds__(kReduceThreads) void sp2_bounds_kernel(const double* __restrict__ F, int n,
+                                                                    int chunks, const double* __restrict__ partial,
+                                                                    double* __restrict__ state) {
+  __shared__ double sh[kReduceThreads];
+  const int tid = threadIdx.x;
+  double lo = INFINITY, hi = -INFINITY, tr = 0.0;
+  for (int c = tid; c < n; c += kReduceThreads) {
+    double rad = 0.0;
+    for (int y = 0; y < chunks; ++y) rad += partial[static_cast<std::size_t>(y) * n + c];
+    const double d = F[static_cast<std::size_t>(c) * n + c];
+    lo = fmin(lo, d - rad);
+    hi = fmax(hi, d + rad);
+    tr += d;
+  }
+  sh[tid] = lo;
+  __syncthreads();
+  for (int off = kReduceThreads / 2; off > 0; off >>= 1) {
+    if (tid < off) sh[tid] = fmin(sh[tid], sh[tid + off]);
+    __syncthreads();
+  }
+  lo = sh[0];
+  __syncthreads();
+  sh[tid] = hi;
+  __syncthreads();
+  for (int off = kReduceThreads / 2; off > 0; off >>= 1) {
+    if (tid < off) sh[tid] = fmax(sh[tid], sh[tid + off]);
+    __syncthreads();
+  }
+  hi = sh[0];
+  __syncthreads();
+  tr = block_sum(tr, sh);
+  if (tid == 0) {
+    const double width = hi - lo;
+    const bool degenerate = !(width > 1e-12 * fmax(1.0, fmax(fabs(lo), fabs(hi))));
+    state[0] = degenerate ? 0.0 : (n * hi - tr) / width;
+    state[1] = 0.0;
+    state[2] = degenerate ? 1.0 : 0.0;
+    state[3] = 0.0;
+    state[4] = lo;
+    state[5] = hi;
+    state[6] = 0.0;
+    state[7] = degenerate ? 1.0 : 0.0;
+  }
+}
+
+__global__ void sp2_init_kernel(const double* __restrict__ F, int n, const double* __restrict__ state,
+                                double* __restrict__ X) {
+  const std::size_t e = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<std::size_t>(n) * n) return;
+  const double lo = state[4], hi = state[5];
+  const double width = state[7] != 0.0 ? 1.0 : hi - lo;
+  const bool diag = e / n == e % n;
+  X[e] = ((diag ? hi : 0.0) - F[e]) / width;
+}
+
+constexpr int kSp2StepThreads = 256;
+
+__global__ __launch_bounds__(kSp2StepThreads) void sp2_step_kernel(double* __restrict__ X,
+                                                                   const double* __restrict__ P, int n,
+                                                                   double nocc, double* __restrict__ state,
+                                                                   int k) {
+  __shared__ double sh[kSp2StepThreads];
+  const int tid = threadIdx.x;
+  const int cur = k & 1, nxt = cur ^ 1;
+  if (state[2 + cur] != 0.0) {
+    if (blockIdx.x == 0 && tid == 0) {
+      state[nxt] = state[cur];
+      state[2 + nxt] = 1.0;
+      state[8 + k] = 0.0;
+    }
+    return;
+  }
+  // tr P in a fixed order (identical in every block).
+  double t = 0.0;
+  for (int i = tid; i < n; i += kSp2StepThreads) t += P[static_cast<std::size_t>(i) * n + i];
+  sh[tid] = t;
+  __syncthreads();
+  for (int off = kSp2StepThreads / 2; off > 0; off >>= 1) {
+    if (tid < off) sh[tid] += sh[tid + off];
+    __syncthreads();
+  }
+  const double trP = sh[0];
+  const double trX = state[cur];
+  const double err = trX - trP;
+  const bool square = fabs(trP - nocc) < fabs(2.0 * trX - trP - nocc);
+  // Rounding floor: the error has stopped falling over a double step while
+  // already small (in the quadratic regime it falls far below half).
+  const bool freeze = k >= 2 && (err <= 0.0 || (err < 1e-6 && err >= 0.5 * state[8 + k - 2]));
+  const std::size_t count = static_cast<std::size_t>(n) * n;
+  for (std::size_t e = static_cast<std::size_t>(blockIdx.x) * kSp2StepThreads + tid; e < count;
+       e += static_cast<std::size_t>(gridDim.x) * kSp2StepThreads)
+    X[e] = square ? P[e] : 2.0 * X[e] - P[e];
+  if (blockIdx.x == 0 && tid == 0) {
+    state[nxt] = square ? trP : 2.0 * trX - trP;
+    state[2 + nxt] = freeze ? 1.0 : 0.0;
+    state[6] = static_cast<double>(k + 1);
+    state[8 + k] = err;
+  }
+}
+
+__global__ __launch_bounds__(64) void sp2_validate_kernel(const double* __restrict__ L, int m,
+                                                          const int* __restrict__ info,
+                                                          const double* __restrict__ state, int steps,
+                                                          double nocc, int* __restrict__ status) {
+  const int lane = threadIdx.x;
+  bool ok = true;
+  for (int i = lane; i < m; i += 64) {
+    const double d = L[static_cast<std::size_t>(i) * m + i];
+    if (!(d >= 0.05)) ok = false;
+  }
+  const bool all = __all(ok);
+  if (lane == 0) {
+    const bool good = all && info[0] == 0 && state[2 + (steps & 1)] != 0.0 && state[7] == 0.0 &&
+                      fabs(state[steps & 1] - nocc) < 1e-6;
+    status[0] = good ? 0 : 1;
+  }
+}
+
+__global__ void copy_kernel(double* __restrict__ dst, const double* __restrict__ src, std::size_t count) {
+  const std::size_t stride = static_cast<std::size_t>(gridDim.x) * blockDim.x;
+  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride)
+    dst[i] = src[i];
+}
+
 unsigned blocks_for(std::size_t count, unsigned threads = 256) {
   return static_cast<unsigned>((count + threads - 1) / threads);
 }
 
 }  // namespace
+
+void copy_device(double* dst, const double* src, std::size_t count, hipStream_t s) {
+  if (!count) return;
+  hipLaunchKernelGGL(copy_kernel, dim3(std::min<unsigned>(blocks_for(count), 8192u)), dim3(256), 0, s, dst, src, count);
+  VQC_LAUNCH("copy_kernel");
+}
 
 void assemble_fock(const double* H, const double* J, const double* K, double cj, double ck, std::size_t count,
                    double* F, hipStream_t s) {
@@ -276,6 +436,46 @@ void check_finite(const double* x, std::size_t count, int* flag, hipStream_t s) 
   hipLaunchKernelGGL(finite_kernel, dim3(std::min<unsigned>(blocks_for(count), 4096u)), dim3(256), 0, s, x, count,
                      flag);
   VQC_LAUNCH("finite_kernel");
+}
+
+std::size_t sp2_partial_size(int n) {
+  return static_cast<std::size_t>((n + kSp2Rows - 1) / kSp2Rows) * n;
+}
+
+void sp2_begin(const double* F, int n, double* partial, double* state, double* X, hipStream_t s) {
+  const int chunks = (n + kSp2Rows - 1) / kSp2Rows;
+  hipLaunchKernelGGL(sp2_radius_kernel, dim3((n + kSp2Cols - 1) / kSp2Cols, chunks), dim3(kSp2Cols * kSp2Lanes), 0,
+                     s, F, n, partial);
+  VQC_LAUNCH("sp2_radius_kernel");
+  hipLaunchKernelGGL(sp2_bounds_kernel, dim3(1), dim3(kReduceThreads), 0, s, F, n, chunks, partial, state);
+  VQC_LAUNCH("sp2_bounds_kernel");
+  const std::size_t e = static_cast<std::size_t>(n) * n;
+  hipLaunchKernelGGL(sp2_init_kernel, dim3(blocks_for(e)), dim3(256), 0, s, F, n, state, X);
+  VQC_LAUNCH("sp2_init_kernel");
+}
+
+void sp2_step(double* X, const double* P, int n, double nocc, double* state, int k, hipStream_t s) {
+  if (k < 0 || k >= kSp2MaxSteps) throw std::invalid_argument("SP2 step index out of range");
+  const std::size_t e = static_cast<std::size_t>(n) * n;
+  const unsigned blocks = std::min<unsigned>(blocks_for(e, kSp2StepThreads * 4), 1024u);
+  hipLaunchKernelGGL(sp2_step_kernel, dim3(std::max(1u, blocks)), dim3(kSp2StepThreads), 0, s, X, P, n, nocc, state,
+                     k);
+  VQC_LAUNCH("sp2_step_kernel");
+}
+
+Sp2Summary sp2_summary(const double* h, int steps) {
+  Sp2Summary out;
+  out.trace = h[steps & 1];
+  out.converged = h[2 + (steps & 1)] != 0.0 && h[7] == 0.0;
+  out.applied = static_cast<int>(h[6]);
+  out.degenerate = h[7] != 0.0;
+  return out;
+}
+
+void sp2_validate(const double* L, int m, const int* info, const double* state, int steps, double nocc,
+                  int* status, hipStream_t s) {
+  hipLaunchKernelGGL(sp2_validate_kernel, dim3(1), dim3(64), 0, s, L, m, info, state, steps, nocc, status);
+  VQC_LAUNCH("sp2_validate_kernel");
 }
 
 void extract_diagonal(const double* A, int n, double* diag, hipStream_t s) {

@@ -5,6 +5,12 @@
 
 namespace vqc {
 
+// dst[i] = src[i] for i < count by a kernel on stream s.  Device-to-device
+// copies between the plan's buffers go through this rather than
+// hipMemcpyAsync, so every producer and consumer of a buffer is an ordinary
+// kernel of the same stream.
+void copy_device(double* dst, const double* src, std::size_t count, hipStream_t s);
+
 // F = H + cj J + ck K.
 void assemble_fock(const double* H, const double* J, const double* K, double cj, double ck, std::size_t count,
                    double* F, hipStream_t s);
@@ -34,5 +40,34 @@ void factor_from_eigen(const double* V, const double* w, const int* idx, int n, 
 void check_finite(const double* x, std::size_t count, int* flag, hipStream_t s);
 // Symmetric copy from a column-major Cholesky factor: diag[i] = L[i + i*n].
 void extract_diagonal(const double* A, int n, double* diag, hipStream_t s);
+
+// Second-order spectral projection (SP2 purification; A. M. N. Niklasson,
+// Phys. Rev. B 66, 155115 (2002)) of a symmetric F (n x n, orthonormal
+// basis): X0 = (hi I - F) / (hi - lo) over F's Gershgorin interval, then
+// X <- X^2 or 2X - X^2, whichever brings tr X closer to nocc, until the
+// idempotency error tr(X - X^2) reaches its rounding floor.  X converges to
+// the projector onto the nocc lowest eigenvectors of F.  The state array
+// (kSp2StateSize doubles) carries the traces, the freeze flag and the
+// error history on the device, so steps need no host round trip.
+constexpr int kSp2MaxSteps = 100;
+constexpr int kSp2StateSize = 8 + kSp2MaxSteps;
+std::size_t sp2_partial_size(int n);
+void sp2_begin(const double* F, int n, double* partial, double* state, double* X, hipStream_t s);
+// Step k (0-based) given P = X X; the step after the error floor is reached
+// and every later one leave X unchanged.
+void sp2_step(double* X, const double* P, int n, double nocc, double* state, int k, hipStream_t s);
+// state[0..7] after `steps` steps: trace, converged flag, applied steps.
+struct Sp2Summary {
+  double trace;
+  bool converged;
+  int applied;
+  bool degenerate;
+};
+Sp2Summary sp2_summary(const double* host_state, int steps);
+// status[0] = 1 unless: info == 0, every diagonal entry of the Cholesky
+// factor L (m x m, column-major) is at least 0.05 and the trace after
+// `steps` steps is within 1e-6 of nocc.
+void sp2_validate(const double* L, int m, const int* info, const double* state, int steps, double nocc,
+                  int* status, hipStream_t s);
 
 }  // namespace vqc
