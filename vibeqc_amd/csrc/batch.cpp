@@ -18,32 +18,21 @@
 namespace vqc {
 namespace {
 
-// One stream-ordered memory pool per worker, kept mapped between plans
-// (release threshold: never) and released as a whole when destroyed.
-hipMemPool_t worker_pool(int device) {
-  hipMemPoolProps props{};
-  props.allocType = hipMemAllocationTypePinned;
-  props.location.type = hipMemLocationTypeDevice;
-  props.location.id = device;
+// Keep freed stream-ordered memory in the device pool for the next plan
+// rather than returning it to the driver at every synchronisation.
+void retain_pool(int device) {
   hipMemPool_t pool = nullptr;
-  VQC_HIP(hipMemPoolCreate(&pool, &props));
+  if (hipDeviceGetDefaultMemPool(&pool, device) != hipSuccess || !pool) return;
   uint64_t threshold = std::numeric_limits<uint64_t>::max();
   (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
-  return pool;
 }
 
-// Routes this thread's device allocations to a worker's stream and pool for
-// the lifetime of the object.
+// Routes this thread's device allocations to a worker's stream for the
+// lifetime of the object.
 class AllocationScope {
  public:
-  AllocationScope(hipStream_t s, hipMemPool_t p) {
-    allocation_stream() = s;
-    allocation_pool() = p;
-  }
-  ~AllocationScope() {
-    allocation_stream() = nullptr;
-    allocation_pool() = nullptr;
-  }
+  explicit AllocationScope(hipStream_t s) { allocation_stream() = s; }
+  ~AllocationScope() { allocation_stream() = nullptr; }
   AllocationScope(const AllocationScope&) = delete;
   AllocationScope& operator=(const AllocationScope&) = delete;
 };
@@ -95,6 +84,7 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
     throw std::invalid_argument("batch coordinate count mismatch");
   if (forces.size() != count) throw std::invalid_argument("batch force request count mismatch");
   VQC_HIP(hipSetDevice(device_));
+  retain_pool(device_);
   std::vector<BatchItemOutcome> out(count);
   // Worker count: the option, else VIBEQC_BATCH_WORKERS, else 8.
   int requested = opts_.workers;
@@ -105,18 +95,15 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
   // after all plans that borrow them.
   std::vector<hipStream_t> streams(workers, nullptr);
   std::vector<rocblas_handle> handles(workers, nullptr);
-  std::vector<hipMemPool_t> pools(workers, nullptr);
   auto release_resources = [&]() {
     for (int w = 0; w < workers; ++w) {
       if (streams[w]) (void)hipStreamSynchronize(streams[w]);
       if (handles[w]) rocblas_destroy_handle(handles[w]);
       if (streams[w]) (void)hipStreamDestroy(streams[w]);
-      if (pools[w]) (void)hipMemPoolDestroy(pools[w]);
     }
   };
   try {
     for (int w = 0; w < workers; ++w) {
-      pools[w] = worker_pool(device_);
       VQC_HIP(hipStreamCreateWithFlags(&streams[w], hipStreamNonBlocking));
       VQC_BLAS(rocblas_create_handle(&handles[w]));
       VQC_BLAS(rocblas_set_stream(handles[w], streams[w]));
@@ -141,7 +128,7 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
     const int w = static_cast<int>(k % workers);
     BatchItemOutcome& r = out[i];
     r.bucket = bucket_[i];
-    AllocationScope scope(streams[w], pools[w]);
+    AllocationScope scope(streams[w]);
     try {
       System sys = systems_[i];
       System aux = aux_[i];
@@ -187,7 +174,7 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
       for (std::size_t k = w0; k < w1; ++k) setup_item(k);
       auto run = [&](int w) {
         if (hipSetDevice(device_) != hipSuccess) return;
-        AllocationScope scope(streams[w], pools[w]);
+        AllocationScope scope(streams[w]);
         for (std::size_t k = w0 + static_cast<std::size_t>(w); k < w1; k += workers) solve_item(k);
         (void)hipStreamSynchronize(streams[w]);
       };
@@ -201,7 +188,7 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
         if (!plans[k] || r.error || !r.scf.converged) continue;
         try {
           if (forces[i]) {
-            AllocationScope scope(streams[k % workers], pools[k % workers]);
+            AllocationScope scope(streams[k % workers]);
             r.scf.forces = plans[k]->converged_forces();
           }
           if (opts_.warm_starts && warm_updates_) warm_[i] = r.scf.density;
@@ -209,7 +196,6 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
           r.error = std::current_exception();
         }
       }
-      // Plans free into the pool of the worker that allocated them.
       for (std::size_t k = w0; k < w1; ++k) plans[k].reset();
     }
   } catch (...) {

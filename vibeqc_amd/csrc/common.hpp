@@ -115,14 +115,6 @@ inline hipStream_t& allocation_stream() {
   static thread_local hipStream_t stream = nullptr;
   return stream;
 }
-// Memory pool of the stream-ordered allocations above (null: the device's
-// default pool).  Batch workers each own one, so memory freed on one
-// worker's stream is never handed to another's and every worker's sequence
-// of addresses depends only on its own work.
-inline hipMemPool_t& allocation_pool() {
-  static thread_local hipMemPool_t pool = nullptr;
-  return pool;
-}
 
 // Owned device allocation; moves, never copies.
 template <class T>
@@ -150,12 +142,8 @@ class DeviceArray {
     if (n) {
       void* q = nullptr;
       s_ = allocation_stream();
-      if (s_ && allocation_pool())
-        hip_check(hipMallocFromPoolAsync(&q, n * sizeof(T), allocation_pool(), s_), "hipMallocFromPoolAsync");
-      else if (s_)
-        hip_check(hipMallocAsync(&q, n * sizeof(T), s_), "hipMallocAsync");
-      else
-        hip_check(hipMalloc(&q, n * sizeof(T)), "hipMalloc");
+      if (s_) hip_check(hipMallocAsync(&q, n * sizeof(T), s_), "hipMallocAsync");
+      else hip_check(hipMalloc(&q, n * sizeof(T)), "hipMalloc");
       p_ = static_cast<T*>(q);
       n_ = n;
       if (poison_allocations()) {

@@ -212,8 +212,11 @@ __global__ __launch_bounds__(128 * WQ) void occupied_u_kernel(const T* __restric
   constexpr int CP = (BK * BI + NT - 1) / NT;  // C elements staged per thread
   __shared__ T Bs[2][BK][BQ];
   __shared__ T Cs[2][BK][BI];
-  const int mu = blockIdx.x / nqb;
-  const int q0 = (blockIdx.x % nqb) * BQ;
+  // Q-tile-major order: the blocks in flight share one Q tile, so the
+  // second gather of each packed row (as (mu, nu) and as (nu, mu)) finds it
+  // in the last-level cache instead of HBM.
+  const int mu = blockIdx.x % n;
+  const int q0 = (blockIdx.x / n) * BQ;
   const int i0 = blockIdx.y * BI;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wq = wave % WQ, wi = wave / WQ;
@@ -324,7 +327,7 @@ __global__ __launch_bounds__(128 * WQ) void occupied_u_kernel(const T* __restric
 // chunk per plane.  C^T arrives pre-split (ldc rows of ldct K-contiguous
 // entries, zero beyond n).  U is written as hi/lo planes.  The next k-step is
 // staged in registers while the current one runs on the MFMAs.
-template <int NIW, int WQ>
+template <int NIW, int WQ, int NBUF>
 __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __restrict__ B,
                                                                 const __bf16* __restrict__ Cth,
                                                                 const __bf16* __restrict__ Ctl,
@@ -339,10 +342,17 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
   constexpr int WI = 16 / WQ, XF = BQ / (16 * WQ), YF = BI / (16 * WI);
   static_assert(YF * 16 * WI == BI, "i tile must split evenly over the waves");
   constexpr int YP = (BI * 4 + NT - 1) / NT;  // 16-byte C chunks per thread and plane
-  __shared__ __bf16 Xh[BQ][BK + PAD], Xl[BQ][BK + PAD];
-  __shared__ __bf16 Yh[BI][BK + PAD], Yl[BI][BK + PAD];
-  const int mu = blockIdx.x / nqb;
-  const int q0 = (blockIdx.x % nqb) * BQ;
+  // NBUF = 2: the waves that finish the MFMAs of step k store step k + 1
+  // into the other LDS buffer while the rest are still multiplying, with one
+  // barrier per step.  NBUF = 1 (wide i tiles, whose two buffers would not
+  // fit beside the rho row): store after a barrier, then a second barrier.
+  __shared__ __bf16 Xh[NBUF][BQ][BK + PAD], Xl[NBUF][BQ][BK + PAD];
+  __shared__ __bf16 Yh[NBUF][BI][BK + PAD], Yl[NBUF][BI][BK + PAD];
+  // Q-tile-major order: the blocks in flight share one Q tile, so the
+  // second gather of each packed row (as (mu, nu) and as (nu, mu)) finds it
+  // in the last-level cache instead of HBM.
+  const int mu = blockIdx.x % n;
+  const int q0 = (blockIdx.x / n) * BQ;
   const int i0 = blockIdx.y * BI;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wq = wave % WQ, wi = wave / WQ;
@@ -351,33 +361,57 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
   for (int x = 0; x < XF; ++x)
 #pragma unroll
     for (int y = 0; y < YF; ++y) acc[x][y] = f4{};
-  const int xq = tid % BQ, xg = tid / BQ;  // one Q column, 8 consecutive nu
-  float rx[8];
+  // One Q column and 8 consecutive nu per thread.  BQ is a multiple of the
+  // wave size, so the nu group is uniform across a wave: every gathered row
+  // address is computed once per wave in scalar registers and each lane adds
+  // only its Q offset.  Lanes past ldq (the last Q tile) read column 0 and
+  // keep a zero.
+  static_assert(BQ % 64 == 0, "a wave must stay inside one nu group");
+  const int xq = tid % BQ;
+  const int xg = __builtin_amdgcn_readfirstlane(tid / BQ);
+  const bool q_ok = q0 + xq < ldq;
+  const int q_off = q_ok ? q0 + xq : 0;
   const bool with_rho = rho_part != nullptr && blockIdx.y == 0;
   double racc = 0.0;
-  int rnu0 = 0;
   extern __shared__ double drow[];  // D[mu][:], staged once when rho is formed
   if (with_rho)
     for (int k = tid; k < n; k += NT) drow[k] = D[static_cast<std::size_t>(mu) * n + k];
   __syncthreads();
-  // Only the B gather (HBM) is staged in registers; the small C^T tile is
-  // L2-resident and is copied straight into LDS.  The first i-block also
-  // accumulates rho_Q partials from the staged B values (FP32 B, FP64 sums)
-  // when they are consumed in sstore, so the loads stay in flight.
+  // Only the B gather (HBM) is staged in registers, one step ahead; the small
+  // C^T tile is L2-resident and is copied straight into LDS.  The first
+  // i-block also accumulates rho_Q partials from the staged B values (FP32
+  // B, FP64 sums) when they are consumed in sstore, so the loads stay in
+  // flight.
+  // The eight rows of a step are gathered without branches, so all eight
+  // loads are in flight together: the row pointer starts at nu = min(nb,
+  // n - 1) and advances in scalar registers by one row below the diagonal
+  // (rows (mu, nu) are consecutive) and by nu rows above it (row (nu, mu)
+  // follows (nu - 1, mu) after nu rows); rows past n repeat the last valid
+  // one and are zeroed when stored.
+  float rx[8];
   auto gload = [&](int nu0) {
-    const int q = q0 + xq;
+    const int nb = nu0 + xg * 8;
+    const float* row = B + packed_index(mu, nb < n ? nb : n - 1) * ldq;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const int nu = nu0 + xg * 8 + e;
-      rx[e] = (nu < n && q < ldq) ? B[packed_index(mu, nu) * ldq + q] : 0.0f;
+      rx[e] = row[q_off];
+      if (e < 7) {
+        // Branch-free step (a branch here would make the compiler wait for
+        // each load at the join).
+        const int next = nb + e + 1;
+        const int step = next < n ? (next <= mu ? 1 : next) : 0;
+        row += static_cast<std::size_t>(step) * ldq;
+      }
     }
-    rnu0 = nu0;
   };
-  auto sstore = [&](int nu0) {
+  auto sstore = [&](int buf, int nu0) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      if (nu0 + xg * 8 + e >= n || !q_ok) rx[e] = 0.0f;
     if (with_rho) {
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        const int nu = rnu0 + xg * 8 + e;
+        const int nu = nu0 + xg * 8 + e;
         if (nu < n) racc += drow[nu] * static_cast<double>(rx[e]);
       }
     }
@@ -389,36 +423,36 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
       h[e] = hh;
       l[e] = ll;
     }
-    *reinterpret_cast<bf8*>(&Xh[xq][xg * 8]) = h;
-    *reinterpret_cast<bf8*>(&Xl[xq][xg * 8]) = l;
+    *reinterpret_cast<bf8*>(&Xh[buf][xq][xg * 8]) = h;
+    *reinterpret_cast<bf8*>(&Xl[buf][xq][xg * 8]) = l;
 #pragma unroll
     for (int j = 0; j < YP; ++j) {
       const int e = tid + j * NT;
       if (e < BI * 4) {
         const std::size_t off = static_cast<std::size_t>(i0 + (e >> 2)) * ldct + nu0 + (e & 3) * 8;
-        *reinterpret_cast<bf8*>(&Yh[e >> 2][(e & 3) * 8]) = *reinterpret_cast<const bf8*>(Cth + off);
-        *reinterpret_cast<bf8*>(&Yl[e >> 2][(e & 3) * 8]) = *reinterpret_cast<const bf8*>(Ctl + off);
+        *reinterpret_cast<bf8*>(&Yh[buf][e >> 2][(e & 3) * 8]) = *reinterpret_cast<const bf8*>(Cth + off);
+        *reinterpret_cast<bf8*>(&Yl[buf][e >> 2][(e & 3) * 8]) = *reinterpret_cast<const bf8*>(Ctl + off);
       }
     }
   };
   gload(0);
-  sstore(0);
+  sstore(0, 0);
+  if (BK < n) gload(BK);
   __syncthreads();
+  int cur = 0;
   for (int nu0 = 0; nu0 < n; nu0 += BK) {
-    const bool more = nu0 + BK < n;
-    if (more) gload(nu0 + BK);
     const int kq = 8 * (lane >> 4);
     bf8 ah[XF], al[XF];
 #pragma unroll
     for (int x = 0; x < XF; ++x) {
-      ah[x] = *reinterpret_cast<const bf8*>(&Xh[wq * 16 * XF + x * 16 + (lane & 15)][kq]);
-      al[x] = *reinterpret_cast<const bf8*>(&Xl[wq * 16 * XF + x * 16 + (lane & 15)][kq]);
+      ah[x] = *reinterpret_cast<const bf8*>(&Xh[cur][wq * 16 * XF + x * 16 + (lane & 15)][kq]);
+      al[x] = *reinterpret_cast<const bf8*>(&Xl[cur][wq * 16 * XF + x * 16 + (lane & 15)][kq]);
     }
 #pragma unroll
     for (int y = 0; y < YF; ++y) {
       const int row = wi * 16 * YF + y * 16 + (lane & 15);
-      const bf8 bh = *reinterpret_cast<const bf8*>(&Yh[row][kq]);
-      const bf8 bl = *reinterpret_cast<const bf8*>(&Yl[row][kq]);
+      const bf8 bh = *reinterpret_cast<const bf8*>(&Yh[cur][row][kq]);
+      const bf8 bl = *reinterpret_cast<const bf8*>(&Yl[cur][row][kq]);
       // XF independent accumulators between dependent MFMAs keep the pipe full.
 #pragma unroll
       for (int x = 0; x < XF; ++x) acc[x][y] = mfma(ah[x], bh, acc[x][y]);
@@ -427,14 +461,25 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
 #pragma unroll
       for (int x = 0; x < XF; ++x) acc[x][y] = mfma(al[x], bh, acc[x][y]);
     }
-    __syncthreads();
-    if (more) {
-      sstore(nu0 + BK);
+    if constexpr (NBUF == 2) {
+      if (nu0 + BK < n) {
+        // The other buffer was last read before the previous barrier.
+        sstore(cur ^ 1, nu0 + BK);
+        if (nu0 + 2 * BK < n) gload(nu0 + 2 * BK);
+      }
       __syncthreads();
+      cur ^= 1;
+    } else {
+      __syncthreads();
+      if (nu0 + BK < n) {
+        sstore(0, nu0 + BK);
+        if (nu0 + 2 * BK < n) gload(nu0 + 2 * BK);
+        __syncthreads();
+      }
     }
   }
   if (with_rho) {
-    double* red = reinterpret_cast<double*>(&Xh[0][0]);  // NT doubles fit in Xh
+    double* red = reinterpret_cast<double*>(&Xh[0][0][0]);  // NT doubles fit in Xh
     red[xg * BQ + xq] = racc;
     __syncthreads();
     if (tid < BQ && q0 + tid < ldq) {
@@ -755,7 +800,7 @@ void launch_u(Precision prec, const void* B, const OccupiedOperands& ops, int ld
     dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (32 * NIW));
     // 8 x 2 waves: the 4 x 4 layout needs more than the 128 registers a
     // 16-wave block allows once NIW >= 8.
-    hipLaunchKernelGGL((occupied_u_bf16x3_kernel<NIW, 8>), grid, dim3(1024), dyn, s, static_cast<const float*>(B),
+    hipLaunchKernelGGL((occupied_u_bf16x3_kernel<NIW, 8, (NIW <= 4 ? 2 : 1)>), grid, dim3(1024), dyn, s, static_cast<const float*>(B),
                        static_cast<const __bf16*>(ops.ct_hi), static_cast<const __bf16*>(ops.ct_lo),
                        static_cast<__bf16*>(U), static_cast<__bf16*>(U) + static_cast<std::size_t>(ops.n) * ops.r * ldq,
                        ops.n, ldq, ops.ldct, ops.r, nqb, D, rho_part);

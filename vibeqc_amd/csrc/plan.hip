@@ -558,29 +558,26 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   }
   scale_columns_inv_sqrt(V.data(), w.data(), n, W.data(), stream_);
   gemm(rocblas_operation_none, rocblas_operation_transpose, n, W.data(), n, V.data(), n, X.data());
-  // New density from F.  The occupied space of F~ = X F X comes from a
-  // diagonalisation (first iteration, after a rejected purification and for
-  // the final canonical orbitals) or, otherwise, from SP2 purification of F~
-  // followed by one subspace step from the previous occupied orbitals Cx:
-  // Q = P Cx, Cx <- Q chol(Q^T Q)^-T, which spans the same space as the
-  // lowest nocc eigenvectors and is orthonormal.  The purification is GEMMs
-  // and elementwise kernels, where the eigensolver is a long chain of small
-  // launches (its tridiagonal reduction alone is most of an SCF iteration
-  // for a few hundred functions).
-  const bool sp2_usable = nocc > 0 && nocc < n;
-  const std::size_t nx = sp2_usable ? static_cast<std::size_t>(n) * nocc : 1;
-  DeviceArray<double> Cx(nx), Qb(nx), Mb(sp2_usable ? static_cast<std::size_t>(nocc) * nocc : 1),
-      sp2_state(kSp2StateSize), sp2_partial(sp2_partial_size(n));
+  // New density from F.  The occupied space of F~ = X F X comes from SP2
+  // purification of F~ followed by a pivoted Cholesky factor G of the
+  // projector (P = G G^T; the factor of D is sqrt(2) X G), or from a
+  // diagonalisation for the final canonical orbitals and whenever the
+  // purification does not reach an integer trace (a Fermi level without a
+  // gap) or the factor is rank deficient.  The purification is GEMMs and
+  // elementwise kernels, where the eigensolver is a long chain of small
+  // launches (its tridiagonal reduction alone is most of an SCF iteration for
+  // a few hundred functions).
+  const bool sp2_usable = nocc > 0 && nocc < n && pivoted_cholesky_fits(n, nocc);
+  DeviceArray<double> G(sp2_usable ? static_cast<std::size_t>(n) * nocc : 1), sp2_state(kSp2StateSize),
+      sp2_partial(sp2_partial_size(n));
   DeviceArray<int> sp2_status(1);
-  DeviceArray<rocblas_int> minfo(1);
-  bool have_cx = false, check_sp2 = false;
   int sp2_steps = 0;
   double sp2_lo = 0.0, sp2_hi = 0.0;
   const double dnocc = static_cast<double>(nocc);
   // Purify T2 into V; false when it does not reach an integer trace.
   auto purify = [&]() {
     sp2_begin(T2.data(), n, sp2_partial.data(), sp2_state.data(), V.data(), stream_);
-    int steps = 0, batch = sp2_hint_ > 0 ? sp2_hint_ : 24;
+    int steps = 0, batch = sp2_hint_ > 0 ? sp2_hint_ : 32;
     double hstate[8];
     for (;;) {
       for (int j = 0; j < batch && steps < kSp2MaxSteps; ++j, ++steps) {
@@ -600,35 +597,30 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
       batch = 4;
     }
   };
+  // Returns true when the density came from purification; its factor's
+  // status is then checked by the caller (purified_ok) before use.
   auto new_density = [&](const double* Fm, double* Dout, bool canonical) {
     gemm(rocblas_operation_none, rocblas_operation_none, n, Fm, n, X.data(), n, T1.data());
     gemm(rocblas_operation_none, rocblas_operation_none, n, X.data(), n, T1.data(), n, T2.data());
-    check_sp2 = false;
-    if (!canonical && have_cx && purify()) {
-      VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, n, nocc, n, &one, V.data(), n,
-                             Cx.data(), n, &zero, Qb.data(), n));
-      VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, nocc, nocc, n, &one,
-                             Qb.data(), n, Qb.data(), n, &zero, Mb.data(), nocc));
-      VQC_BLAS(rocsolver_dpotrf(blas_, rocblas_fill_lower, nocc, Mb.data(), nocc, minfo.data()));
-      VQC_BLAS(rocblas_dtrsm(blas_, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
-                             rocblas_diagonal_non_unit, n, nocc, &one, Mb.data(), nocc, Qb.data(), n));
-      sp2_validate(Mb.data(), nocc, minfo.data(), sp2_state.data(), sp2_steps, dnocc, sp2_status.data(), stream_);
-      std::swap(Cx, Qb);
+    bool purified = false;
+    if (!canonical && sp2_usable && purify()) {
+      pivoted_cholesky(V.data(), n, nocc, G.data(), sp2_status.data(), stream_);
       VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, n, nocc, n, &one, X.data(), n,
-                             Cx.data(), n, &zero, W.data(), n));
-      check_sp2 = true;
+                             G.data(), n, &zero, W.data(), n));
+      purified = true;
     } else {
       eigh(T2.data(), V.data(), w.data());
       gemm(rocblas_operation_none, rocblas_operation_none, n, X.data(), n, V.data(), n, W.data());
-      if (sp2_usable) {
-        copy_device(Cx.data(), V.data(), nx, stream_);
-        have_cx = true;
-      }
     }
     occupied_from_orbitals(W.data(), n, nocc, ldc_, std::sqrt(2.0), c64_.data(), stream_);
     VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, n, n, std::max(nocc, 1), &one,
                            c64_.data(), ldc_, c64_.data(), ldc_, &zero, Dout, n));
-    return check_sp2;
+    return purified;
+  };
+  auto purified_ok = [&]() {
+    int status = 1;
+    sp2_status.download(&status, 1, stream_);
+    return status == 0;
   };
   ScfOutcome out;
   // Initial density: core Hamiltonian guess (src/scf/initial_guess/density.cpp:635-672) or caller seed.
@@ -637,7 +629,7 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
     // The seed's factor: eigen-decompose D (positive part) so K can use the occupied path.
     const int packed = factorise_density(D_.data());
     if ((packed >> 16) != (packed & 0xffff)) throw std::invalid_argument("initial density must be positive semidefinite");
-  } else {
+  } else if (new_density(H_.data(), D_.data(), false) && !purified_ok()) {
     new_density(H_.data(), D_.data(), true);
   }
   int rank_now = initial_density ? -1 : nocc;
@@ -666,7 +658,7 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   double previous = std::numeric_limits<double>::infinity();
   bool converged = false;
   unsigned it = 0;
-  double energy = 0.0, de = 0.0, rms = 0.0, last_rms = std::numeric_limits<double>::infinity();
+  double energy = 0.0, de = 0.0, rms = 0.0;
   for (it = 1; it <= o.max_iterations; ++it) {
     const Precision p = in_reduced ? prec_ : Precision::FP64;
     int r = nocc, rp = nocc;
@@ -706,32 +698,25 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
         Fuse = Fe.data();
       }
     }
-    // Purify once the density has settled (RMS change below 1e-2): earlier,
-    // the occupied space turns too far between iterations for one subspace
-    // step and the iteration would diagonalise after all.
-    const bool purified = new_density(Fuse, Dn.data(), !(last_rms < 1e-2));
+    const bool purified = new_density(Fuse, Dn.data(), false);
     rank_now = nocc;
     rms_difference(Dn.data(), D_.data(), nn, scalars.data() + 2, stream_);
     double hs[4];
     scalars.download(hs, 4, stream_);
     if (purified) {
-      int status = 1;
-      sp2_status.download(&status, 1, stream_);
-      if (status != 0) {
-        // Rejected subspace step (rotated too far, or a near-degenerate
-        // Fermi level): this iteration diagonalises instead.
+      if (purified_ok()) {
+        ++out.purified_iterations;
+      } else {
+        // Rank-deficient projector: this iteration diagonalises instead.
         new_density(Fuse, Dn.data(), true);
         rms_difference(Dn.data(), D_.data(), nn, scalars.data() + 2, stream_);
         scalars.download(hs, 4, stream_);
-      } else {
-        ++out.purified_iterations;
       }
     }
     sync();
     energy = 0.5 * (hs[0] + hs[1]) + enuc_;
     rms = hs[2];
     if (!std::isfinite(energy) || !std::isfinite(rms)) throw NumericalFailure("non-finite SCF iterate");
-    last_rms = rms;
     if (trace_scf)
       std::fprintf(stderr,
                    "{\"trace\": \"scf\", \"enuc\": %.17g, \"it\": %u, \"e1\": %.17g, \"e2\": %.17g, \"rms\": %.17g, "

@@ -336,22 +336,75 @@ __global__ __launch_bounds__(kSp2StepThreads) void sp2_step_kernel(double* __res
   }
 }
 
-__global__ __launch_bounds__(64) void sp2_validate_kernel(const double* __restrict__ L, int m,
-                                                          const int* __restrict__ info,
-                                                          const double* __restrict__ state, int steps,
-                                                          double nocc, int* __restrict__ status) {
-  const int lane = threadIdx.x;
-  bool ok = true;
-  for (int i = lane; i < m; i += 64) {
-    const double d = L[static_cast<std::size_t>(i) * m + i];
-    if (!(d >= 0.05)) ok = false;
+constexpr int kCholThreads = 1024;
+constexpr int kCholMaxLds = 8000;  // doubles of dynamic LDS (d[n], pivot row gp[r]), under 64 KB
+
+__global__ __launch_bounds__(kCholThreads) void pivoted_cholesky_kernel(const double* __restrict__ P, int n, int r,
+                                                                        double* __restrict__ G,
+                                                                        int* __restrict__ status) {
+  extern __shared__ double dyn[];
+  double* d = dyn;       // remaining diagonal
+  double* gp = dyn + n;  // row `pivot` of G, columns < k
+  __shared__ double wave_v[kCholThreads / 64];
+  __shared__ int wave_i[kCholThreads / 64];
+  __shared__ int pivot;
+  __shared__ double pivot_v;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int i = tid; i < n; i += kCholThreads) d[i] = P[static_cast<std::size_t>(i) * n + i];
+  __syncthreads();
+  for (int k = 0; k < r; ++k) {
+    // Arg-max of d (lowest index on ties).
+    double bv = -INFINITY;
+    int bi = n;
+    for (int i = tid; i < n; i += kCholThreads)
+      if (d[i] > bv) {
+        bv = d[i];
+        bi = i;
+      }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_xor(bv, off);
+      const int oi = __shfl_xor(bi, off);
+      if (ov > bv || (ov == bv && oi < bi)) {
+        bv = ov;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      wave_v[wave] = bv;
+      wave_i[wave] = bi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double v = wave_v[0];
+      int i0 = wave_i[0];
+      for (int w = 1; w < kCholThreads / 64; ++w)
+        if (wave_v[w] > v || (wave_v[w] == v && wave_i[w] < i0)) {
+          v = wave_v[w];
+          i0 = wave_i[w];
+        }
+      pivot = i0;
+      pivot_v = v;
+    }
+    __syncthreads();
+    const int p = pivot;
+    const double pv = pivot_v;
+    if (!(pv >= 1e-8) || p >= n) {
+      if (tid == 0) status[0] = 1;
+      return;
+    }
+    for (int j = tid; j < k; j += kCholThreads) gp[j] = G[p + static_cast<std::size_t>(j) * n];
+    __syncthreads();
+    const double scale = 1.0 / sqrt(pv);
+    for (int i = tid; i < n; i += kCholThreads) {
+      double v = P[i + static_cast<std::size_t>(p) * n];
+      for (int j = 0; j < k; ++j) v -= G[i + static_cast<std::size_t>(j) * n] * gp[j];
+      v *= scale;
+      G[i + static_cast<std::size_t>(k) * n] = v;
+      d[i] = i == p ? 0.0 : d[i] - v * v;
+    }
+    __syncthreads();
   }
-  const bool all = __all(ok);
-  if (lane == 0) {
-    const bool good = all && info[0] == 0 && state[2 + (steps & 1)] != 0.0 && state[7] == 0.0 &&
-                      fabs(state[steps & 1] - nocc) < 1e-6;
-    status[0] = good ? 0 : 1;
-  }
+  if (tid == 0) status[0] = 0;
 }
 
 __global__ void copy_kernel(double* __restrict__ dst, const double* __restrict__ src, std::size_t count) {
@@ -472,10 +525,13 @@ Sp2Summary sp2_summary(const double* h, int steps) {
   return out;
 }
 
-void sp2_validate(const double* L, int m, const int* info, const double* state, int steps, double nocc,
-                  int* status, hipStream_t s) {
-  hipLaunchKernelGGL(sp2_validate_kernel, dim3(1), dim3(64), 0, s, L, m, info, state, steps, nocc, status);
-  VQC_LAUNCH("sp2_validate_kernel");
+bool pivoted_cholesky_fits(int n, int r) { return r >= 1 && r <= n && n + r <= kCholMaxLds; }
+
+void pivoted_cholesky(const double* P, int n, int r, double* G, int* status, hipStream_t s) {
+  if (!pivoted_cholesky_fits(n, r)) throw std::invalid_argument("pivoted Cholesky size out of range");
+  const std::size_t lds = static_cast<std::size_t>(n + r) * sizeof(double);
+  hipLaunchKernelGGL(pivoted_cholesky_kernel, dim3(1), dim3(kCholThreads), lds, s, P, n, r, G, status);
+  VQC_LAUNCH("pivoted_cholesky_kernel");
 }
 
 void extract_diagonal(const double* A, int n, double* diag, hipStream_t s) {

@@ -64,10 +64,13 @@ struct Sp2Summary {
   bool degenerate;
 };
 Sp2Summary sp2_summary(const double* host_state, int steps);
-// status[0] = 1 unless: info == 0, every diagonal entry of the Cholesky
-// factor L (m x m, column-major) is at least 0.05 and the trace after
-// `steps` steps is within 1e-6 of nocc.
-void sp2_validate(const double* L, int m, const int* info, const double* state, int steps, double nocc,
-                  int* status, hipStream_t s);
+// G (n x r, column-major, leading dimension n) with P = G G^T for a
+// symmetric positive semidefinite P of rank r, such as a converged SP2
+// projector: r steps of Cholesky with diagonal pivoting in one workgroup
+// (pivot: largest remaining diagonal, lowest index on ties).  status[0] = 1
+// when a pivot falls below 1e-8, i.e. P is not of rank r.  Requires
+// pivoted_cholesky_fits(n, r).
+bool pivoted_cholesky_fits(int n, int r);
+void pivoted_cholesky(const double* P, int n, int r, double* G, int* status, hipStream_t s);
 
 }  // namespace vqc
