@@ -502,6 +502,158 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
     }
 }
 
+// BF16X3 U with the B operand taken straight from registers (i tiles up to
+// 96 columns).  Lane l of a wave gathers the adjacent Q rows q + 2 (l & 15)
+// and q + 2 (l & 15) + 1 (one 8-byte load) at the eight consecutive nu of k
+// group l >> 4: the even rows form one A fragment of
+// v_mfma_f32_16x16x32_bf16 and the odd rows another, each in exactly the
+// operand layout, split into hi/lo in registers; B never passes through LDS.
+// Only the C^T tile (i x nu) is staged in LDS, double-buffered, one barrier
+// per k-step.  8 waves x 32 Q rows per block (two blocks per CU).  The C^T
+// global loads of the next step are issued before its B gathers, so the wait
+// that precedes their LDS store never waits for the gathers.
+template <int NIW>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void occupied_u_bf16x3_direct_kernel(
+    const float* __restrict__ B, const __bf16* __restrict__ Cth, const __bf16* __restrict__ Ctl,
+    __bf16* __restrict__ Uh, __bf16* __restrict__ Ul, int n, int ldq, int ldct, int r,
+    const double* __restrict__ D, double* __restrict__ rho_part) {
+  constexpr int NW = 8, NT = 64 * NW, BQ = 32 * NW, BK = 32, BI = 32 * NIW, PAD = 8;
+  constexpr int XF = 2, YF = BI / 16;
+  constexpr int YP = (BI * 4 + NT - 1) / NT;  // 16-byte C^T chunks per thread and plane
+  __shared__ __bf16 Yh[2][BI][BK + PAD], Yl[2][BI][BK + PAD];
+  extern __shared__ double drow[];  // D[mu][:], staged once when rho is formed
+  const int mu = blockIdx.x % n;    // Q-tile-major block order
+  const int q0 = (blockIdx.x / n) * BQ;
+  const int i0 = blockIdx.y * BI;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int kg = lane >> 4;  // k group: nu offset 8 kg within a step
+  const int qa = q0 + wave * 32 + 2 * (lane & 15);  // rows qa (x = 0) and qa + 1 (x = 1)
+  const bool q_ok = qa < ldq;                        // ldq is even
+  const int q_off = q_ok ? qa : 0;
+  const bool with_rho = rho_part != nullptr && blockIdx.y == 0;
+  if (with_rho)
+    for (int k = tid; k < n; k += NT) drow[k] = D[static_cast<std::size_t>(mu) * n + k];
+  f4 acc[XF][YF];
+#pragma unroll
+  for (int x = 0; x < XF; ++x)
+#pragma unroll
+    for (int y = 0; y < YF; ++y) acc[x][y] = f4{};
+  double racc[XF] = {0.0, 0.0};
+  float2 rx[8];
+  bf8 rch[YP], rcl[YP];
+  // Rows of packed B: start at nu = min(nb, n - 1), then one row below the
+  // diagonal and nu rows above it; rows past n repeat the last valid one and
+  // are zeroed when converted.  No branch between the loads.
+  auto gload = [&](int nu0) {
+    const int nb = nu0 + 8 * kg;
+    const float* row = B + packed_index(mu, nb < n ? nb : n - 1) * ldq;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      rx[j] = *reinterpret_cast<const float2*>(row + q_off);
+      if (j < 7) {
+        const int next = nb + j + 1;
+        const int step = next < n ? (next <= mu ? 1 : next) : 0;
+        row += static_cast<std::size_t>(step) * ldq;
+      }
+    }
+  };
+  auto cload = [&](int nu0) {
+#pragma unroll
+    for (int jj = 0; jj < YP; ++jj) {
+      const int e = tid + jj * NT;
+      if (e < BI * 4) {
+        const std::size_t off = static_cast<std::size_t>(i0 + (e >> 2)) * ldct + nu0 + (e & 3) * 8;
+        rch[jj] = *reinterpret_cast<const bf8*>(Cth + off);
+        rcl[jj] = *reinterpret_cast<const bf8*>(Ctl + off);
+      }
+    }
+  };
+  auto cstore = [&](int buf) {
+#pragma unroll
+    for (int jj = 0; jj < YP; ++jj) {
+      const int e = tid + jj * NT;
+      if (e < BI * 4) {
+        *reinterpret_cast<bf8*>(&Yh[buf][e >> 2][(e & 3) * 8]) = rch[jj];
+        *reinterpret_cast<bf8*>(&Yl[buf][e >> 2][(e & 3) * 8]) = rcl[jj];
+      }
+    }
+  };
+  cload(0);
+  gload(0);
+  cstore(0);
+  __syncthreads();
+  int cur = 0;
+  for (int nu0 = 0; nu0 < n; nu0 += BK) {
+    // This step's A fragments (waits for its gathers).
+    bf8 ah[XF], al[XF];
+    const int nb = nu0 + 8 * kg;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const bool ok = nb + j < n && q_ok;
+      const float v0 = ok ? rx[j].x : 0.0f, v1 = ok ? rx[j].y : 0.0f;
+      if (with_rho && nb + j < n) {
+        const double d = drow[nb + j];
+        racc[0] += d * static_cast<double>(v0);
+        racc[1] += d * static_cast<double>(v1);
+      }
+      __bf16 hh, ll;
+      split_bf16(v0, hh, ll);
+      ah[0][j] = hh;
+      al[0][j] = ll;
+      split_bf16(v1, hh, ll);
+      ah[1][j] = hh;
+      al[1][j] = ll;
+    }
+    const bool more = nu0 + BK < n;
+    if (more) {
+      cload(nu0 + BK);
+      gload(nu0 + BK);
+    }
+    const int kq = 8 * kg;
+#pragma unroll
+    for (int y = 0; y < YF; ++y) {
+      const int row = y * 16 + (lane & 15);
+      const bf8 bh = *reinterpret_cast<const bf8*>(&Yh[cur][row][kq]);
+      const bf8 bl = *reinterpret_cast<const bf8*>(&Yl[cur][row][kq]);
+#pragma unroll
+      for (int x = 0; x < XF; ++x) acc[x][y] = mfma(ah[x], bh, acc[x][y]);
+#pragma unroll
+      for (int x = 0; x < XF; ++x) acc[x][y] = mfma(ah[x], bl, acc[x][y]);
+#pragma unroll
+      for (int x = 0; x < XF; ++x) acc[x][y] = mfma(al[x], bh, acc[x][y]);
+    }
+    // The other buffer was last read before the previous barrier.
+    if (more) cstore(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
+  }
+  if (with_rho) {
+    // Sum the four k groups (lanes l, l ^ 16, l ^ 32, l ^ 48) of each row.
+#pragma unroll
+    for (int x = 0; x < XF; ++x) {
+      double t = racc[x];
+      t += __shfl_xor(t, 16);
+      t += __shfl_xor(t, 32);
+      if (kg == 0 && q_ok) rho_part[static_cast<std::size_t>(mu) * ldq + qa + x] = t;
+    }
+  }
+  // Fragment x row m holds Q = q0 + 32 wave + 2 m + x; a lane's accumulator
+  // element v is row m = 4 (l >> 4) + v of column i = l & 15.
+  const int qw = q0 + wave * 32;
+  if (qw >= ldq) return;
+#pragma unroll
+  for (int y = 0; y < YF; ++y) {
+    const int i = i0 + y * 16 + (lane & 15);
+    if (i >= r) continue;
+    const std::size_t off = (static_cast<std::size_t>(mu) * r + i) * ldq + qw + 8 * (lane >> 4);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      split_bf16(acc[0][y][v], Uh[off + 2 * v], Ul[off + 2 * v]);
+      split_bf16(acc[1][y][v], Uh[off + 2 * v + 1], Ul[off + 2 * v + 1]);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // K, step 2: lower-triangular split-K SYRK over W (n x kd, K-contiguous rows).
 // Block: 256 threads = 2x2 waves, tile 64 x 64, partial results per split.
@@ -795,6 +947,13 @@ void launch_u(Precision prec, const void* B, const OccupiedOperands& ops, int ld
     dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (32 * NIW));
     hipLaunchKernelGGL((occupied_u_kernel<float, NIW, WQ>), grid, dim3(128 * WQ), dyn, s, static_cast<const float*>(B),
                        ops.c32, static_cast<float*>(U), ops.n, ldq, ops.ldc, ops.r, nqb, D, rho_part);
+  } else if constexpr (NIW <= 3) {
+    const int nqb = (ldq + 255) / 256;
+    dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (32 * NIW));
+    hipLaunchKernelGGL((occupied_u_bf16x3_direct_kernel<NIW>), grid, dim3(512), dyn, s, static_cast<const float*>(B),
+                       static_cast<const __bf16*>(ops.ct_hi), static_cast<const __bf16*>(ops.ct_lo),
+                       static_cast<__bf16*>(U), static_cast<__bf16*>(U) + static_cast<std::size_t>(ops.n) * ops.r * ldq,
+                       ops.n, ldq, ops.ldct, ops.r, D, rho_part);
   } else {
     const int nqb = (ldq + kUTileQ - 1) / kUTileQ;
     dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (32 * NIW));
