@@ -102,7 +102,7 @@ def _time_precisions(plan, nocc, steps, warmup, precisions=PRECISIONS):
 # HIP events on the plan's stream around exactly these launches).
 PHASE_KERNELS = {
     "exchange_u_ms": ("occupied_u", {"fp64": "occupied_u_kernel<double", "fp32": "occupied_u_kernel<float",
-                                     "bf16x3": "occupied_u_bf16x3_kernel"}),
+                                     "bf16x3": "occupied_u_bf16x3"}),
     "exchange_syrk_ms": ("exchange_syrk", {"fp64": "exchange_syrk_kernel<double", "fp32": "exchange_syrk_kernel<float",
                                            "bf16x3": "exchange_syrk_bf16x3_kernel"}),
     "coulomb_ms": ("coulomb_matrix", {"fp64": "coulomb_matrix_kernel<double", "fp32": "coulomb_matrix_kernel<float",
@@ -150,7 +150,7 @@ def _roofline(phase_ms, n, naux, npair, nocc, precision, traffic=None):
     sym = symbols[precision]
     measured = (traffic or {}).get(sym)
     return {
-        "kernel": sym,
+        "kernel": measured.get("kernel_name", sym) if isinstance(measured, dict) else sym,
         "phase": label,
         "bound": bound,
         "achieved": achieved,
@@ -182,6 +182,7 @@ def _pmc_traffic(workload, precision, copy_mib=256, timeout=240):
         return {"error": "rocprofv3 not found"}
     per_counter = {}
     scale = {}
+    names = {}
     env = dict(os.environ, TMPDIR="/tmp")
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         out = tempfile.mkdtemp(prefix="vqc_pmc_", dir="/tmp")
@@ -208,6 +209,7 @@ def _pmc_traffic(workload, precision, copy_mib=256, timeout=240):
             v = [float(r["Counter_Value"]) for r in rows if sym in r["Kernel_Name"]]
             if v:
                 vals[sym] = sum(v) / len(v) * 1024.0 * scale[counter]
+                names[sym] = next(r["Kernel_Name"] for r in rows if sym in r["Kernel_Name"])
         per_counter[counter] = vals
     traffic = {}
     for sym in per_counter["FETCH_SIZE"]:
@@ -217,6 +219,7 @@ def _pmc_traffic(workload, precision, copy_mib=256, timeout=240):
             "write_bytes": per_counter["WRITE_SIZE"].get(sym, 0.0),
             "fetch_scale": scale["FETCH_SIZE"],
             "write_scale": scale["WRITE_SIZE"],
+            "kernel_name": names.get(sym, sym).replace("(anonymous namespace)::", "").replace("void ", "").split("(")[0],
         }
     return traffic
 
