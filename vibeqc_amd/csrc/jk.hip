@@ -20,6 +20,8 @@ namespace {
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+typedef float fl2 __attribute__((ext_vector_type(2)));
 
 template <class T>
 struct AccOf;
@@ -68,6 +70,22 @@ __device__ __forceinline__ void unpack_pair(std::size_t p, int& mu, int& nu) {
 __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
   hi = static_cast<__bf16>(x);
   lo = static_cast<__bf16>(x - static_cast<float>(hi));
+}
+
+// Split of eight floats into bf16 hi/lo fragments, two at a time (one packed
+// conversion per pair for hi and one for lo).
+__device__ __forceinline__ void split8_bf16(const float (&x)[8], bf8& hi, bf8& lo) {
+  bf2 h[4], l[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const fl2 v = {x[2 * k], x[2 * k + 1]};
+    h[k] = __builtin_convertvector(v, bf2);
+    l[k] = __builtin_convertvector(v - __builtin_convertvector(h[k], fl2), bf2);
+  }
+  hi = __builtin_shufflevector(__builtin_shufflevector(h[0], h[1], 0, 1, 2, 3), __builtin_shufflevector(h[2], h[3], 0, 1, 2, 3),
+                               0, 1, 2, 3, 4, 5, 6, 7);
+  lo = __builtin_shufflevector(__builtin_shufflevector(l[0], l[1], 0, 1, 2, 3), __builtin_shufflevector(l[2], l[3], 0, 1, 2, 3),
+                               0, 1, 2, 3, 4, 5, 6, 7);
 }
 
 // ---------------------------------------------------------------------------
@@ -521,41 +539,37 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void o
   constexpr int XF = 2, YF = BI / 16;
   constexpr int YP = (BI * 4 + NT - 1) / NT;  // 16-byte C^T chunks per thread and plane
   __shared__ __bf16 Yh[2][BI][BK + PAD], Yl[2][BI][BK + PAD];
-  extern __shared__ double drow[];  // D[mu][:], staged once when rho is formed
-  const int mu = blockIdx.x % n;    // Q-tile-major block order
+  // Dynamic LDS: element offsets of the packed rows (mu, nu) for nu < n + BK
+  // (rows past n repeat row n - 1, so gathers need no bounds test), then, for
+  // the rho pass, D[mu][nu] as float.
+  extern __shared__ double dyn[];
+  std::size_t* rowoff = reinterpret_cast<std::size_t*>(dyn);
+  float* drow = reinterpret_cast<float*>(rowoff + n + BK);
+  const int mu = blockIdx.x % n;  // Q-tile-major block order
   const int q0 = (blockIdx.x / n) * BQ;
   const int i0 = blockIdx.y * BI;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int kg = lane >> 4;  // k group: nu offset 8 kg within a step
-  const int qa = q0 + wave * 32 + 2 * (lane & 15);  // rows qa (x = 0) and qa + 1 (x = 1)
-  const bool q_ok = qa < ldq;                        // ldq is even
-  const int q_off = q_ok ? qa : 0;
+  const int qw = q0 + wave * 32;
+  const bool q_ok = qw < ldq;  // wave-uniform: ldq is a multiple of 32
+  const int q_off = q_ok ? qw + 2 * (lane & 15) : 0;  // rows q_off (x = 0), q_off + 1 (x = 1)
   const bool with_rho = rho_part != nullptr && blockIdx.y == 0;
+  for (int k = tid; k < n + BK; k += NT) rowoff[k] = packed_index(mu, k < n ? k : n - 1) * ldq;
   if (with_rho)
-    for (int k = tid; k < n; k += NT) drow[k] = D[static_cast<std::size_t>(mu) * n + k];
+    for (int k = tid; k < n; k += NT) drow[k] = static_cast<float>(D[static_cast<std::size_t>(mu) * n + k]);
   f4 acc[XF][YF];
 #pragma unroll
   for (int x = 0; x < XF; ++x)
 #pragma unroll
     for (int y = 0; y < YF; ++y) acc[x][y] = f4{};
   double racc[XF] = {0.0, 0.0};
-  float2 rx[8];
+  fl2 rx[8];
   bf8 rch[YP], rcl[YP];
-  // Rows of packed B: start at nu = min(nb, n - 1), then one row below the
-  // diagonal and nu rows above it; rows past n repeat the last valid one and
-  // are zeroed when converted.  No branch between the loads.
+  const float* Bq = B + q_off;
   auto gload = [&](int nu0) {
     const int nb = nu0 + 8 * kg;
-    const float* row = B + packed_index(mu, nb < n ? nb : n - 1) * ldq;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      rx[j] = *reinterpret_cast<const float2*>(row + q_off);
-      if (j < 7) {
-        const int next = nb + j + 1;
-        const int step = next < n ? (next <= mu ? 1 : next) : 0;
-        row += static_cast<std::size_t>(step) * ldq;
-      }
-    }
+    for (int j = 0; j < 8; ++j) rx[j] = *reinterpret_cast<const fl2*>(Bq + rowoff[nb + j]);
   };
   auto cload = [&](int nu0) {
 #pragma unroll
@@ -578,32 +592,42 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void o
       }
     }
   };
+  __syncthreads();  // rowoff and drow
   cload(0);
   gload(0);
   cstore(0);
   __syncthreads();
   int cur = 0;
   for (int nu0 = 0; nu0 < n; nu0 += BK) {
-    // This step's A fragments (waits for its gathers).
-    bf8 ah[XF], al[XF];
+    // This step's A fragments (waits for its gathers).  Lanes past ldq and
+    // nu past n (only in the last step) contribute zeros.
     const int nb = nu0 + 8 * kg;
+    float v0[8], v1[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const bool ok = nb + j < n && q_ok;
-      const float v0 = ok ? rx[j].x : 0.0f, v1 = ok ? rx[j].y : 0.0f;
-      if (with_rho && nb + j < n) {
-        const double d = drow[nb + j];
-        racc[0] += d * static_cast<double>(v0);
-        racc[1] += d * static_cast<double>(v1);
-      }
-      __bf16 hh, ll;
-      split_bf16(v0, hh, ll);
-      ah[0][j] = hh;
-      al[0][j] = ll;
-      split_bf16(v1, hh, ll);
-      ah[1][j] = hh;
-      al[1][j] = ll;
+      v0[j] = rx[j].x;
+      v1[j] = rx[j].y;
     }
+    if (!q_ok || nu0 + BK > n) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (!q_ok || nb + j >= n) v0[j] = v1[j] = 0.0f;
+    }
+    if (with_rho) {
+      // rho partials: FP32 over the step's eight nu, FP64 across steps.
+      float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const float d = nb + j < n ? drow[nb + j] : 0.0f;
+        s0 = fmaf(d, v0[j], s0);
+        s1 = fmaf(d, v1[j], s1);
+      }
+      racc[0] += static_cast<double>(s0);
+      racc[1] += static_cast<double>(s1);
+    }
+    bf8 ah[XF], al[XF];
+    split8_bf16(v0, ah[0], al[0]);
+    split8_bf16(v1, ah[1], al[1]);
     const bool more = nu0 + BK < n;
     if (more) {
       cload(nu0 + BK);
@@ -634,13 +658,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void o
       double t = racc[x];
       t += __shfl_xor(t, 16);
       t += __shfl_xor(t, 32);
-      if (kg == 0 && q_ok) rho_part[static_cast<std::size_t>(mu) * ldq + qa + x] = t;
+      if (kg == 0 && q_ok) rho_part[static_cast<std::size_t>(mu) * ldq + q_off + x] = t;
     }
   }
-  // Fragment x row m holds Q = q0 + 32 wave + 2 m + x; a lane's accumulator
-  // element v is row m = 4 (l >> 4) + v of column i = l & 15.
-  const int qw = q0 + wave * 32;
-  if (qw >= ldq) return;
+  // Fragment x row m holds Q = qw + 2 m + x; a lane's accumulator element v
+  // is row m = 4 (l >> 4) + v of column i = l & 15.
+  if (!q_ok) return;
 #pragma unroll
   for (int y = 0; y < YF; ++y) {
     const int i = i0 + y * 16 + (lane & 15);
@@ -716,6 +739,8 @@ __global__ __launch_bounds__(512) void exchange_syrk_bf16x3_kernel(const __bf16*
   int rb, cb;
   syrk_tile_of(tile, ncb, 1, rb, cb);
   const int m0 = rb * T, l0 = cb * T;
+  // A diagonal tile multiplies a panel by itself: one gather, one LDS copy.
+  const bool diag = m0 == l0;
   const std::size_t k0 = kbegin + static_cast<std::size_t>(split) * kchunk;
   const std::size_t k1 = k0 + kchunk < kend ? k0 + kchunk : kend;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -739,7 +764,7 @@ __global__ __launch_bounds__(512) void exchange_syrk_bf16x3_kernel(const __bf16*
         ra[j][0] = *reinterpret_cast<const bf8*>(Wh + off);
         ra[j][1] = *reinterpret_cast<const bf8*>(Wl + off);
       }
-      if (gb < n) {
+      if (!diag && gb < n) {
         const std::size_t off = static_cast<std::size_t>(gb) * kd + kb + kc;
         rbv[j][0] = *reinterpret_cast<const bf8*>(Wh + off);
         rbv[j][1] = *reinterpret_cast<const bf8*>(Wl + off);
@@ -752,8 +777,10 @@ __global__ __launch_bounds__(512) void exchange_syrk_bf16x3_kernel(const __bf16*
       const int c = tid + j * NT, row = c >> 2, kc = (c & 3) << 3;
       *reinterpret_cast<bf8*>(&Ah[row][kc]) = ra[j][0];
       *reinterpret_cast<bf8*>(&Al[row][kc]) = ra[j][1];
-      *reinterpret_cast<bf8*>(&Bh[row][kc]) = rbv[j][0];
-      *reinterpret_cast<bf8*>(&Bl[row][kc]) = rbv[j][1];
+      if (!diag) {
+        *reinterpret_cast<bf8*>(&Bh[row][kc]) = rbv[j][0];
+        *reinterpret_cast<bf8*>(&Bl[row][kc]) = rbv[j][1];
+      }
     }
   };
   gload(k0);
@@ -766,8 +793,9 @@ __global__ __launch_bounds__(512) void exchange_syrk_bf16x3_kernel(const __bf16*
     bf8 bh[YF], bl[YF];
 #pragma unroll
     for (int j = 0; j < YF; ++j) {
-      bh[j] = *reinterpret_cast<const bf8*>(&Bh[wc * 16 * YF + j * 16 + (lane & 15)][kq]);
-      bl[j] = *reinterpret_cast<const bf8*>(&Bl[wc * 16 * YF + j * 16 + (lane & 15)][kq]);
+      const int row = wc * 16 * YF + j * 16 + (lane & 15);
+      bh[j] = *reinterpret_cast<const bf8*>(diag ? &Ah[row][kq] : &Bh[row][kq]);
+      bl[j] = *reinterpret_cast<const bf8*>(diag ? &Al[row][kq] : &Bl[row][kq]);
     }
 #pragma unroll
     for (int i = 0; i < XF; ++i) {
@@ -950,7 +978,10 @@ void launch_u(Precision prec, const void* B, const OccupiedOperands& ops, int ld
   } else if constexpr (NIW <= 3) {
     const int nqb = (ldq + 255) / 256;
     dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (32 * NIW));
-    hipLaunchKernelGGL((occupied_u_bf16x3_direct_kernel<NIW>), grid, dim3(512), dyn, s, static_cast<const float*>(B),
+    // Row-offset table (n + 32 entries) and the rho row as float.
+    const std::size_t ddyn = static_cast<std::size_t>(ops.n + 32) * sizeof(std::size_t) +
+                             (rho_part ? static_cast<std::size_t>(ops.n) * sizeof(float) : 0);
+    hipLaunchKernelGGL((occupied_u_bf16x3_direct_kernel<NIW>), grid, dim3(512), ddyn, s, static_cast<const float*>(B),
                        static_cast<const __bf16*>(ops.ct_hi), static_cast<const __bf16*>(ops.ct_lo),
                        static_cast<__bf16*>(U), static_cast<__bf16*>(U) + static_cast<std::size_t>(ops.n) * ops.r * ldq,
                        ops.n, ldq, ops.ldct, ops.r, D, rho_part);
