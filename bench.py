@@ -16,8 +16,11 @@ Also reported:
   * water64: (H2O)64 at liquid density (12.4 A cube), cc-pVDZ (1536 AO) /
     cc-pVDZ-JKFIT (7424), 320 occupied -- the large single system the
     mixed/FP64 speedup target is stated on, with its own roofline and CPU sample;
+  * water64.scf: the (H2O)64 SCF to 1e-10 Eh / 1e-8 RMS in FP64 and in mixed
+    precision (BF16X3 iterations, then FP64 refinement), seconds and energy gap;
+  * ragged_batch: 512 (H2O)n clusters, n ~ U{1..16}, molecules/s mixed and FP64;
   * scf_water4: (H2O)4 cc-pVDZ SCF seconds per molecule in FP64 and in mixed
-    precision (BF16X3 iterations, then FP64 refinement) with the energy gap.
+    precision with the energy gap.
 
 Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu-baseline]
 """
@@ -332,7 +335,16 @@ def _ragged_cpu_baseline(sample=4, max_waters=6):
     }
 
 
+def _scf_stats(plan, result, seconds):
+    st = plan.last_solve_stats()
+    return {"energy": result.energy, "iterations": result.iterations, "fock_builds": result.fock_builds,
+            "reduced_iterations": st["reduced_iterations"], "purified_densities": st["purified_densities"],
+            "scf_s": seconds}
+
+
 def _scf_timing(precision):
+    """(H2O)4 SCF on a fresh plan, timed after one untimed solve (first
+    launches of the SCF kernels)."""
     from vibeqc_amd import AOBasis, FockPlan, shells_for
     from vibeqc_amd.geometry import water_cluster
 
@@ -342,10 +354,20 @@ def _scf_timing(precision):
     t0 = time.perf_counter()
     with FockPlan(orb, auxiliary=aux, precision=precision) as plan:
         t1 = time.perf_counter()
-        r = plan.solve(energy_tolerance=1e-10, density_tolerance=1e-8)
+        plan.solve(energy_tolerance=1e-10, density_tolerance=1e-8)
         t2 = time.perf_counter()
-    return {"energy": r.energy, "iterations": r.iterations, "fock_builds": r.fock_builds, "setup_s": t1 - t0,
-            "scf_s": t2 - t1, "total_s": t2 - t0}
+        r = plan.solve(energy_tolerance=1e-10, density_tolerance=1e-8)
+        out = _scf_stats(plan, r, time.perf_counter() - t2)
+    out["setup_s"] = t1 - t0
+    return out
+
+
+def _scf_on_plan(plan, precision):
+    """Full SCF (core guess, 1e-10 Eh / 1e-8 RMS) on an existing plan."""
+    plan.set_precision(precision)
+    t0 = time.perf_counter()
+    r = plan.solve(energy_tolerance=1e-10, density_tolerance=1e-8)
+    return _scf_stats(plan, r, time.perf_counter() - t0)
 
 
 def _profile_run(args):
@@ -374,6 +396,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 PMC byte-count runs")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-molecule ragged batch")
+    ap.add_argument("--no-scf64", action="store_true", help="skip the (H2O)64 FP64 and mixed SCF runs")
     ap.add_argument(
         "--only",
         choices=("c20h42", "water64"),
@@ -411,6 +434,12 @@ def main():
     wn, wnaux, wnpair = wshape["nbf"], wshape["naux"], wshape["npair"]
     _log(f"(H2O)64 plan: nbf {wn}, naux {wnaux}, setup {wsetup:.2f} s")
     w64 = _time_precisions(wplan, wnocc, args.steps, args.warmup)
+    w64_scf = None
+    if not args.no_scf64:
+        w64_scf = {"fp64": _scf_on_plan(wplan, "fp64"), "mixed": _scf_on_plan(wplan, "bf16x3")}
+        w64_scf["energy_diff_mixed_minus_fp64"] = w64_scf["mixed"]["energy"] - w64_scf["fp64"]["energy"]
+        w64_scf["mixed_speedup_vs_fp64"] = w64_scf["fp64"]["scf_s"] / w64_scf["mixed"]["scf_s"]
+        _log(f"(H2O)64 SCF: {w64_scf}")
     wplan.close()
 
     cpu = cpu64 = None
@@ -473,6 +502,7 @@ def main():
             "roofline": _roofline(w64["bf16x3"]["phase_ms"], wn, wnaux, wnpair, wnocc, "bf16x3", traffic64),
             "roofline_fp64": _roofline(w64["fp64"]["phase_ms"], wn, wnaux, wnpair, wnocc, "fp64"),
             "cpu_baseline": cpu64,
+            "scf": w64_scf,
         },
         "ragged_batch": ragged,
         "scf_water4": {

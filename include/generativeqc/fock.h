@@ -196,6 +196,13 @@ GENERATIVEQC_API generativeqc_status vibeqc_fock_plan_one_electron(const generat
 GENERATIVEQC_API generativeqc_status vibeqc_fock_plan_shape(const generativeqc_fock_plan* plan,
                                                             uint64_t* shape6);
 
+/* Extension (no counterpart in the reference): statistics of the plan's last
+ * generativeqc_fock_plan_solve: [0] SCF iterations, [1] of them with
+ * reduced-precision Fock builds, [2] new densities obtained by purification
+ * (the rest by diagonalisation). */
+GENERATIVEQC_API generativeqc_status vibeqc_fock_plan_last_solve_stats(const generativeqc_fock_plan* plan,
+                                                                       uint64_t* stats3);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,6 +5,8 @@
 #include "batch.hpp"
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
 #include <atomic>
 #include <cmath>
 #include <cstdlib>
@@ -168,10 +170,17 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
       r.error = std::current_exception();
     }
   };
+  // VIBEQC_TRACE set: one JSON line per wave with its phase times.
+  const bool trace = std::getenv("VIBEQC_TRACE") != nullptr;
+  using clock = std::chrono::steady_clock;
+  auto ms_since = [](clock::time_point t) { return std::chrono::duration<double, std::milli>(clock::now() - t).count(); };
   try {
     for (std::size_t w0 = 0; w0 < count; w0 += wave) {
       const std::size_t w1 = std::min(count, w0 + wave);
+      const auto t_setup = clock::now();
       for (std::size_t k = w0; k < w1; ++k) setup_item(k);
+      const double setup_ms = ms_since(t_setup);
+      const auto t_scf = clock::now();
       auto run = [&](int w) {
         if (hipSetDevice(device_) != hipSuccess) return;
         AllocationScope scope(streams[w]);
@@ -182,6 +191,8 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
       for (int w = 1; w < workers; ++w) pool.emplace_back(run, w);
       run(0);
       for (auto& t : pool) t.join();
+      const double scf_ms = ms_since(t_scf);
+      const auto t_forces = clock::now();
       for (std::size_t k = w0; k < w1; ++k) {
         const std::size_t i = order_[k];
         BatchItemOutcome& r = out[i];
@@ -196,7 +207,13 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
           r.error = std::current_exception();
         }
       }
+      const double forces_ms = ms_since(t_forces);
       for (std::size_t k = w0; k < w1; ++k) plans[k].reset();
+      if (trace)
+        std::fprintf(stderr,
+                     "{\"trace\": \"batch_wave\", \"first\": %zu, \"systems\": %zu, \"setup_ms\": %.3f, "
+                     "\"scf_ms\": %.3f, \"forces_ms\": %.3f}\n",
+                     w0, w1 - w0, setup_ms, scf_ms, forces_ms);
     }
   } catch (...) {
     plans.clear();

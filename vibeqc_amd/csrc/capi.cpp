@@ -46,6 +46,7 @@ struct generativeqc_fock_plan {
   generativeqc_fock_spec resolved{};
   double screening = 1e-12, threshold = 1e-10;
   uint64_t budget = 0;
+  uint64_t solve_stats[3] = {0, 0, 0};  // iterations, reduced-precision iterations, purified densities
   std::string detail;
 };
 
@@ -430,6 +431,9 @@ generativeqc_status generativeqc_fock_plan_solve(generativeqc_fock_plan* plan,
       finite(seed);
     }
     const vqc::ScfOutcome r = p.solve(opts, initial_density ? &seed : nullptr, output->forces != nullptr);
+    plan->solve_stats[0] = r.iterations;
+    plan->solve_stats[1] = r.reduced_iterations;
+    plan->solve_stats[2] = r.purified_iterations;
     if (!r.converged) {
       plan->detail = "Fock SCF did not converge in " + std::to_string(r.iterations) + " iterations";
       return GENERATIVEQC_STATUS_NOT_CONVERGED;
@@ -570,6 +574,12 @@ generativeqc_status vibeqc_fock_plan_shape(const generativeqc_fock_plan* plan, u
   shape6[3] = static_cast<uint64_t>(p.rank());
   shape6[4] = p.cholesky() ? 1 : 0;
   shape6[5] = static_cast<uint64_t>(p.device_bytes());
+  return GENERATIVEQC_STATUS_SUCCESS;
+}
+
+generativeqc_status vibeqc_fock_plan_last_solve_stats(const generativeqc_fock_plan* plan, uint64_t* stats3) {
+  if (!plan || !stats3) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  std::copy(plan->solve_stats, plan->solve_stats + 3, stats3);
   return GENERATIVEQC_STATUS_SUCCESS;
 }
 

@@ -647,8 +647,8 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   const bool trace_scf = trace_env && std::strcmp(trace_env, "scf") == 0;
   // Hand-over to FP64: BF16X3 Fock matrices carry ~16 significant bits and
   // its stage stops at 1e-4 Eh / 1e-3 RMS density change (FP32: 1e-6 /
-  // 1e-5), or when the density change has not halved its best for three
-  // iterations.  Later hand-overs cost more reduced iterations than the FP64
+  // 1e-5), or when, below 1e-2 RMS, the density change has not halved its
+  // best for three iterations.  Later hand-overs cost more reduced iterations than the FP64
   // refinement saves (measured on water clusters of 1-16 monomers).
   const double e_switch = prec_ == Precision::BF16X3 ? 1e-4 : 1e-6;
   const double d_switch = prec_ == Precision::BF16X3 ? 1e-3 : 1e-5;
@@ -727,10 +727,12 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
     if (in_reduced) {
       const bool settled = it > 1 && de < std::max(e_switch, o.energy_tolerance) &&
                            rms < std::max(d_switch, o.density_tolerance);
+      // A stall only counts near the reduced precision's noise floor; the
+      // first iterations from the core guess often fail to halve the change.
       if (rms < 0.5 * best_rms) {
         best_rms = rms;
         stalled = 0;
-      } else {
+      } else if (rms < 1e-2) {
         ++stalled;
       }
       if (settled || stalled >= 3 || it >= o.max_iterations / 2) {

@@ -245,6 +245,14 @@ class FockPlan:
             "device_bytes": out[5],
         }
 
+    def last_solve_stats(self) -> dict:
+        """Statistics of the last solve(): iterations, iterations with
+        reduced-precision Fock builds, new densities from purification."""
+        out = (ct.c_uint64 * 3)()
+        with self._lock:
+            self._check(self._lib.vibeqc_fock_plan_last_solve_stats(self._handle, out))
+        return {"iterations": out[0], "reduced_iterations": out[1], "purified_densities": out[2]}
+
     def one_electron(self):
         """(S, Hcore) as nbf x nbf arrays."""
         n = self.nao
