@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
@@ -107,6 +108,19 @@ inline bool poison_allocations() {
   return on;
 }
 
+// Debug aid: VIBEQC_GUARD_ALLOC=1 gives every device allocation a 4 KiB tail
+// filled with 0xA5 and checks it when the allocation is released, so a kernel
+// that writes past the end of a buffer is reported (to stderr, with the
+// buffer size) instead of silently corrupting its neighbour.
+inline bool guard_allocations() {
+  static const bool on = [] {
+    const char* v = std::getenv("VIBEQC_GUARD_ALLOC");
+    return v && v[0] == '1';
+  }();
+  return on;
+}
+constexpr std::size_t kGuardBytes = 4096;
+
 // Stream used for stream-ordered allocation by DeviceArrays created on this
 // host thread (null: plain hipMalloc / hipFree).  Batch workers set it to
 // their own stream, so the many small plans they create and destroy never
@@ -142,8 +156,13 @@ class DeviceArray {
     if (n) {
       void* q = nullptr;
       s_ = allocation_stream();
-      if (s_) hip_check(hipMallocAsync(&q, n * sizeof(T), s_), "hipMallocAsync");
-      else hip_check(hipMalloc(&q, n * sizeof(T)), "hipMalloc");
+      const std::size_t bytes = n * sizeof(T) + (guard_allocations() ? kGuardBytes : 0);
+      if (s_) hip_check(hipMallocAsync(&q, bytes, s_), "hipMallocAsync");
+      else hip_check(hipMalloc(&q, bytes), "hipMalloc");
+      if (guard_allocations()) {
+        hip_check(hipMemsetAsync(static_cast<char*>(q) + n * sizeof(T), 0xA5, kGuardBytes, s_), "guard");
+        hip_check(hipStreamSynchronize(s_), "guard");
+      }
       p_ = static_cast<T*>(q);
       n_ = n;
       if (poison_allocations()) {
@@ -153,6 +172,19 @@ class DeviceArray {
     }
   }
   void release() {
+    if (p_ && guard_allocations()) {
+      unsigned char tail[kGuardBytes];
+      if (hipStreamSynchronize(s_) == hipSuccess &&
+          hipMemcpy(tail, reinterpret_cast<char*>(p_) + n_ * sizeof(T), kGuardBytes, hipMemcpyDeviceToHost) ==
+              hipSuccess) {
+        for (std::size_t k = 0; k < kGuardBytes; ++k)
+          if (tail[k] != 0xA5) {
+            std::fprintf(stderr, "{\"guard\": \"overrun\", \"bytes\": %zu, \"first_bad_offset\": %zu}\n",
+                         n_ * sizeof(T), k);
+            break;
+          }
+      }
+    }
     if (p_) {
       if (s_) (void)hipFreeAsync(p_, s_);
       else (void)hipFree(p_);
