@@ -7,6 +7,8 @@ Reads (as data only; nothing is imported or run from the reference):
     {water,water-tetramer,water-hexadecamer-2s4}-def2-svp-spherical-reference.json
     -- GPU4PySCF 1.8.1 cc-pVDZ / cc-pVDZ-JKFIT DF-RHF energies, forces and
     geometries (the file names say def2-svp; the content is cc-pVDZ).
+  * benchmarks/results/issue206-practical-auxiliary/identity/{cc-pvdz,cc-pvdz-jkfit}.json
+    -- the reference's checksummed canonical basis records (copied unchanged).
 The libcint scalar known answers of tests/native/test_density_fitting.cpp:468-499
 are copied into libcint_known_answers.json by hand (they are literals in that
 test, not a data file).
@@ -67,6 +69,10 @@ def main(reference: str) -> None:
             )
         }
         (HERE / f"{out}.json").write_text(json.dumps(record, indent=1) + "\n")
+
+    identity = root / "benchmarks/results/issue206-practical-auxiliary/identity"
+    for name in ("cc-pvdz", "cc-pvdz-jkfit"):
+        (HERE / f"canonical_{name}.json").write_bytes((identity / f"{name}.json").read_bytes())
 
 
 if __name__ == "__main__":

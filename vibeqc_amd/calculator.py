@@ -15,7 +15,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .basis import Atom, AOBasis, NUMBERS, atoms_from, shells_for
+from .basis import Atom, AOBasis, CanonicalBasis, NUMBERS, atoms_from, shells_for
 from .fock import FockBuildSpec, FockPlan
 
 _PRECISIONS = {"fp64": "fp64", "fp32": "fp32", "bf16x3": "bf16x3", "mixed": "bf16x3"}
@@ -85,6 +85,8 @@ class Calculator:
         self.basis = basis
         self.auxiliary_basis = auxiliary_basis
         self.device_id = device_id
+        if basis_representation is None and isinstance(basis, CanonicalBasis):
+            basis_representation = basis.representation  # a canonical record states its own
         self.representation = "cartesian" if basis_representation == "cartesian" else "real_spherical"
         self.metric_threshold = density_fitting_relative_threshold
         self.budget = density_fitting_memory_budget_bytes
