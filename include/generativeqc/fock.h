@@ -47,7 +47,9 @@ typedef struct generativeqc_fock_term {
   generativeqc_fock_approximation approximation;
 } generativeqc_fock_term;
 
-/* Reference: fock.h:37-42.  Only RESTRICTED spin is implemented. */
+/* Reference: fock.h:37-42.  RESTRICTED: D includes double occupation, HF is
+ * cJ = 1, cK = -0.5.  UNRESTRICTED: Da and Db have unit occupation, J comes
+ * from Da + Db and K from each spin; HF is cJ = 1, cK = -1. */
 typedef struct generativeqc_fock_spec {
   uint32_t struct_size, abi_version, spec_version;
   generativeqc_fock_spin spin;
@@ -88,7 +90,8 @@ typedef struct generativeqc_fock_result {
 } generativeqc_fock_result;
 
 /* Reference: fock.h:93-95.  The density must be symmetric (K is built from a
- * factor of D); an asymmetric density returns INVALID_ARGUMENT. */
+ * factor of D); an asymmetric density returns INVALID_ARGUMENT.  Unrestricted
+ * plans take the beta density in `beta` and fill exchange_beta / fock_beta. */
 GENERATIVEQC_API generativeqc_status generativeqc_fock_plan_evaluate(
     generativeqc_fock_plan* plan, const double* density, uint64_t density_count, const double* beta,
     uint64_t beta_count, generativeqc_fock_result* result);
@@ -114,7 +117,9 @@ typedef struct generativeqc_fock_scf_result {
 
 /* Reference: fock.h:134-137.  The whole loop (Fock build, DIIS, eigensolve,
  * density) runs on the device; only per-iteration convergence scalars come
- * back to the host.  Forces are not yet provided (force_count must be 0). */
+ * back to the host.  With forces non-null (derivative_order 1) the analytic
+ * forces -dE/dR of the converged energy are returned.  Unrestricted plans run
+ * UHF: initial_density and density are [alpha; beta], 2 nbf^2 values. */
 GENERATIVEQC_API generativeqc_status generativeqc_fock_plan_solve(
     generativeqc_fock_plan* plan, const generativeqc_fock_scf_controls* controls,
     const double* initial_density, uint64_t initial_density_count,

@@ -543,6 +543,124 @@ def df_rhf(problem: DfProblem, **kw) -> ScfResult:
 
 
 # ----------------------------------------------------------------------------
+# Unrestricted HF (density_fitting.cpp:780-801; rhf.cpp:451-463;
+# mean_field_driver.cpp:201-227, 346-488; reference/mean_field.cpp:26-35;
+# initial_guess/density.cpp:602-623)
+# ----------------------------------------------------------------------------
+
+
+def spin_occupations(electrons: int, multiplicity: int):
+    """(n_alpha, n_beta) from the electron count and multiplicity."""
+    excess = multiplicity - 1
+    if excess < 0 or excess > electrons or (electrons + excess) % 2:
+        raise ValueError("electron count and multiplicity do not define integral UHF occupations")
+    return (electrons + excess) // 2, (electrons - excess) // 2
+
+
+def uhf_jk(B: np.ndarray, Da: np.ndarray, Db: np.ndarray):
+    """Raw J of Da + Db and K of each spin density (density_fitting.cpp:780-801)."""
+    J, Ka = jk(B, Da + Db)
+    _, Ka = jk(B, Da)
+    _, Kb = jk(B, Db)
+    return J, Ka, Kb
+
+
+def uhf_energy(Da, Db, H, Fa, Fb) -> float:
+    """1/2 sum_s D_s (H + F_s), without E_nuc (reference/mean_field.cpp:26-35)."""
+    return float(0.5 * np.sum(Da * (H + Fa)) + 0.5 * np.sum(Db * (H + Fb)))
+
+
+@dataclass
+class UhfResult:
+    energy: float
+    density: np.ndarray  # (2, n, n): alpha, beta
+    iterations: int
+    converged: bool
+    energy_change: float
+    density_rms: float
+    orbital_energies: np.ndarray  # (2, n)
+    coefficients: np.ndarray  # (2, n, n)
+    fock_builds: int
+
+
+def uhf_scf(
+    S,
+    H,
+    enuc: float,
+    nalpha: int,
+    nbeta: int,
+    jk_builder,
+    max_iterations: int = 100,
+    diis_history: int = 8,
+    energy_tolerance: float = 1e-10,
+    density_tolerance: float = 1e-8,
+    initial_density=None,
+) -> UhfResult:
+    """Host UHF loop: core guess with the beta HOMO/LUMO pair rotated by 45
+    degrees when the spins differ, one DIIS over stacked (F_a, F_b) and
+    (F_a D_a S - S D_a F_a, F_b D_b S - S D_b F_b), the density change as the
+    RMS over both blocks, then rebuild / diagonalise / rebuild.
+    jk_builder(Da, Db) returns (J, Ka, Kb)."""
+    n = S.shape[0]
+    X = symmetric_orthogonalizer(S)
+    occ = (nalpha, nbeta)
+    if initial_density is None:
+        _, ca = generalized_eigen(H, X)
+        cb = ca.copy()
+        if nalpha != nbeta and 0 < nbeta < n:
+            c = s = 0.7071067811865476
+            i, j = nbeta - 1, nbeta
+            a, b = cb[:, i].copy(), cb[:, j].copy()
+            cb[:, i], cb[:, j] = c * a + s * b, -s * a + c * b
+        D = np.stack([ca[:, :nalpha] @ ca[:, :nalpha].T, cb[:, :nbeta] @ cb[:, :nbeta].T])
+    else:
+        D = np.array(initial_density, dtype=np.float64).reshape(2, n, n)
+    diis = Diis(diis_history)
+    prev = math.inf
+    builds = 0
+    converged = False
+    e = de = rms = math.nan
+
+    def focks(Dm):
+        J, Ka, Kb = jk_builder(Dm[0], Dm[1])
+        return np.stack([H + J - Ka, H + J - Kb])
+
+    def densities(Fm):
+        eps, C = [], []
+        for s in range(2):
+            es, cs = generalized_eigen(Fm[s], X)
+            eps.append(es)
+            C.append(cs)
+        return np.stack([C[s][:, : occ[s]] @ C[s][:, : occ[s]].T for s in range(2)]), np.stack(eps), np.stack(C)
+
+    for it in range(1, max_iterations + 1):
+        F = focks(D)
+        builds += 2
+        e = uhf_energy(D[0], D[1], H, F[0], F[1]) + enuc
+        r = np.stack([F[s] @ D[s] @ S - S @ D[s] @ F[s] for s in range(2)])
+        Fe = diis.update(F, r)
+        Dn, _, _ = densities(Fe)
+        rms = float(np.sqrt(np.mean((Dn - D) ** 2)))
+        de = abs(e - prev) if math.isfinite(prev) else math.inf
+        D = Dn
+        if it > 1 and de < energy_tolerance and rms < density_tolerance:
+            converged = True
+            break
+        prev = e
+    if not converged:
+        return UhfResult(e, D, it, False, de, rms, np.array([]), np.array([]), builds)
+    D, eps, C = densities(focks(D))
+    F = focks(D)
+    builds += 4
+    energy = uhf_energy(D[0], D[1], H, F[0], F[1]) + enuc
+    return UhfResult(energy, D, it, True, de, rms, eps, C, builds)
+
+
+def df_uhf(problem: DfProblem, nalpha: int, nbeta: int, **kw) -> UhfResult:
+    return uhf_scf(problem.S, problem.H, problem.enuc, nalpha, nbeta, lambda a, b: uhf_jk(problem.B, a, b), **kw)
+
+
+# ----------------------------------------------------------------------------
 # Nuclear gradient of the DF-RHF energy (density_fitting.cpp:898-917, 464-595;
 # density_fitting_metric.cpp:50-99; hf_gradient.cpp:8-18; mean_field.cpp:18-24)
 # ----------------------------------------------------------------------------
@@ -666,6 +784,27 @@ def df_rhf_gradient(atoms_z, atoms_xyz, shells, spherical, aux_shells, aux_spher
     WA, WM = df_weights(problem.A, problem.M, D, Co)
     g = nuclear_repulsion_gradient(atoms_z, atoms_xyz)
     g += grad_one_electron(atoms_z, atoms_xyz, shells, spherical, D, W)
+    g += grad_three_center(atoms_xyz, natom, shells, spherical, aux_shells, aux_spherical, WA)
+    g += grad_metric(atoms_xyz, natom, aux_shells, aux_spherical, WM)
+    return g
+
+
+def df_uhf_gradient(atoms_z, atoms_xyz, shells, spherical, aux_shells, aux_spherical, problem: DfProblem, scf: UhfResult, occ):
+    """dE/dR (natom, 3) of a converged DF-UHF energy (analytic_uhf_forces:
+    one-electron terms of D_a + D_b and of E = sum_s C_s diag(e_s) C_s^T; the
+    DF response of E2 = 1/2 (D_a + D_b) J - 1/2 sum_s D_s K_s)."""
+    if problem.rank != problem.M.shape[0]:
+        raise ValueError("the gradient oracle assumes a full-rank metric")
+    natom = len(atoms_z)
+    D = scf.density
+    W = sum((scf.coefficients[s][:, : occ[s]] * scf.orbital_energies[s][: occ[s]]) @ scf.coefficients[s][:, : occ[s]].T for s in range(2))
+    WA, WM = df_weights_dense(problem.A, problem.M, D[0] + D[1], 1.0, 0.0)
+    for s in range(2):
+        wa, wm = df_weights_dense(problem.A, problem.M, D[s], 0.0, -1.0)
+        WA = WA + wa
+        WM = WM + wm
+    g = nuclear_repulsion_gradient(atoms_z, atoms_xyz)
+    g += grad_one_electron(atoms_z, atoms_xyz, shells, spherical, D[0] + D[1], W)
     g += grad_three_center(atoms_xyz, natom, shells, spherical, aux_shells, aux_spherical, WA)
     g += grad_metric(atoms_xyz, natom, aux_shells, aux_spherical, WM)
     return g

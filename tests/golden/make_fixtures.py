@@ -8,7 +8,10 @@ Reads (as data only; nothing is imported or run from the reference):
     -- GPU4PySCF 1.8.1 cc-pVDZ / cc-pVDZ-JKFIT DF-RHF energies, forces and
     geometries (the file names say def2-svp; the content is cc-pVDZ).
   * benchmarks/results/issue206-practical-auxiliary/identity/{cc-pvdz,cc-pvdz-jkfit}.json
-    -- the reference's checksummed canonical basis records (copied unchanged).
+    -- the reference's checksummed canonical basis records (copied unchanged);
+  * tests/reference_data/validation/hf-plus-uhf.json -- PySCF 2.6.2
+    exact-integral UHF of HF+ (doublet, STO-3G Cartesian): energy, densities,
+    one-electron matrices and orbital energies.
 The libcint scalar known answers of tests/native/test_density_fitting.cpp:468-499
 are copied into libcint_known_answers.json by hand (they are literals in that
 test, not a data file).
@@ -69,6 +72,16 @@ def main(reference: str) -> None:
             )
         }
         (HERE / f"{out}.json").write_text(json.dumps(record, indent=1) + "\n")
+
+    # PySCF 2.6.2 exact-integral UHF of HF+ (doublet, STO-3G Cartesian).
+    meta = json.loads((root / "tests/reference_data/validation/hf-plus-uhf.json").read_text())
+    inp, data = meta["inputs"], meta["data"]
+    record = {
+        "inputs": {k: inp[k] for k in ("atomic_numbers", "coordinates", "shells", "basis_representation", "charge", "multiplicity")},
+        "data": {k: data[k] for k in ("energy", "density", "hcore", "overlap", "nuclear_repulsion", "mo_energy", "mo_occ")},
+        "versions": {"pyscf": meta["provenance"]["pyscf"]},
+    }
+    (HERE / "uhf_hf_plus.json").write_text(json.dumps(record, indent=1) + "\n")
 
     identity = root / "benchmarks/results/issue206-practical-auxiliary/identity"
     for name in ("cc-pvdz", "cc-pvdz-jkfit"):

@@ -1,7 +1,8 @@
-"""RHF ``Calculator`` entry point (host mirror of python/generativeqc/calculator.py:146-200).
+"""HF ``Calculator`` entry point (host mirror of python/generativeqc/calculator.py:146-200).
 
-Only the density-fitted closed-shell RHF path is provided: ``method="rhf"``
-with ``density_fitting`` in ("hip", "cuda", "auto", True).  Coordinates are in
+The density-fitted Hartree-Fock paths are provided: ``method="rhf"``
+(closed shell) and ``method="uhf"`` (any multiplicity; ragged batches are
+RHF only), with ``density_fitting`` in ("hip", "cuda", "auto", True).  Coordinates are in
 Bohr; results carry the energy in Hartree and convergence diagnostics.
 ``precision`` selects the J/K arithmetic: "fp64" (the reference's DF
 precision), "fp32", "bf16x3", or "mixed" (= bf16x3 iterations followed by FP64
@@ -68,8 +69,8 @@ class Calculator:
         screening_tolerance: float | None = None,
         precision: str = "fp64",
     ):
-        if str(method).lower() != "rhf":
-            raise NotImplementedError("the MI355X engine provides density-fitted RHF only")
+        if str(method).lower() not in ("rhf", "uhf"):
+            raise NotImplementedError("the MI355X engine provides density-fitted RHF and UHF only")
         if density_fitting not in ("hip", "cuda", "auto", True):
             raise NotImplementedError("only density-fitted J/K is provided (density_fitting='hip')")
         if device not in ("hip", "cuda", "gpu"):
@@ -81,7 +82,7 @@ class Calculator:
         for v in (energy_tolerance, density_tolerance):
             if not (math.isfinite(v) and v > 0):
                 raise ValueError("tolerances must be finite and positive")
-        self.method = "rhf"
+        self.method = str(method).lower()
         self.basis = basis
         self.auxiliary_basis = auxiliary_basis
         self.device_id = device_id
@@ -119,6 +120,8 @@ class Calculator:
         """generativeqc_method_descriptor of this calculator (generativeqc.h:723-769)."""
         from . import _native
 
+        if self.method != "rhf":
+            raise NotImplementedError("ragged batches run restricted (RHF) systems")
         if self.precision not in ("fp64", "mixed", "bf16x3"):
             raise NotImplementedError("batches run in fp64 or mixed (bf16x3 + FP64 refinement) precision")
         return _native.descriptor(
@@ -153,7 +156,7 @@ class Calculator:
         orbital, aux = self._bases(atoms, charge, multiplicity)
         return FockPlan(
             orbital,
-            FockBuildSpec.hf(),
+            FockBuildSpec.hf("unrestricted" if self.method == "uhf" else "restricted"),
             auxiliary=aux,
             device_id=self.device_id,
             screening_tolerance=self.screening_tolerance,
@@ -176,7 +179,7 @@ class Calculator:
             raise ValueError("properties must include 'energy'")
         unknown = requested - {"energy", "forces"}
         if unknown:
-            raise ValueError(f"method 'rhf' does not support properties: {', '.join(sorted(map(repr, unknown)))}")
+            raise ValueError(f"method {self.method!r} does not support properties: {', '.join(sorted(map(repr, unknown)))}")
         with self.prepare(atoms, charge=charge, multiplicity=multiplicity) as plan:
             r = plan.solve(
                 compute_forces="forces" in requested,

@@ -292,8 +292,8 @@ generativeqc_status generativeqc_fock_plan_create(generativeqc_context* context,
     require(std::isfinite(screening) && screening >= 0.0, "Fock screening tolerance must be finite and nonnegative");
     require(std::isfinite(requested) && requested >= 0.0 && requested < 1.0,
             "Fock metric cutoff must be finite and in [0, 1)");
-    if (spec->spin != GENERATIVEQC_FOCK_RESTRICTED)
-      throw vqc::NotImplemented("only restricted (closed-shell) Fock plans are implemented");
+    require(spec->spin == GENERATIVEQC_FOCK_RESTRICTED || spec->spin == GENERATIVEQC_FOCK_UNRESTRICTED,
+            "unknown Fock spin");
     if (spec->derivative_order > 1)
       throw vqc::NotImplemented("only first nuclear derivatives are implemented");
     if (!term_ok(spec->coulomb) || !term_ok(spec->exchange))
@@ -336,27 +336,39 @@ generativeqc_status generativeqc_fock_plan_evaluate(generativeqc_fock_plan* plan
   plan->detail.clear();
   try {
     auto& p = *plan->plan;
+    const bool unrestricted = plan->resolved.spin == GENERATIVEQC_FOCK_UNRESTRICTED;
     const uint64_t matrix = static_cast<uint64_t>(p.nbf()) * p.nbf();
-    require(density_count == matrix && result->matrix_count == matrix && beta == nullptr && beta_count == 0,
+    require(density_count == matrix && result->matrix_count == matrix &&
+                (unrestricted ? beta != nullptr && beta_count == matrix : beta == nullptr && beta_count == 0),
             "Fock density/output shape or spin mismatch");
     const uint64_t coordinates = static_cast<uint64_t>(p.system().atoms.size()) * 3;
     require(result->gradient ? plan->resolved.derivative_order == 1 && result->gradient_count == coordinates
                              : result->gradient_count == 0,
             "Fock gradient capability/count mismatch");
-    require(result->exchange_beta == nullptr && result->fock_beta == nullptr,
+    require(unrestricted || (result->exchange_beta == nullptr && result->fock_beta == nullptr),
             "restricted plans have no beta outputs");
     const auto bytes = buffer_bytes(matrix);
     disjoint_outputs({{result->coulomb, bytes},
                       {result->exchange_alpha, bytes},
+                      {result->exchange_beta, bytes},
                       {result->fock_alpha, bytes},
+                      {result->fock_beta, bytes},
                       {result->gradient, buffer_bytes(result->gradient_count)},
                       {result, sizeof(*result)}});
     const std::vector<double> d(density, density + matrix);
+    const std::vector<double> db = unrestricted ? std::vector<double>(beta, beta + matrix) : std::vector<double>{};
     std::vector<double> gradient;
-    const vqc::JkResult jk = p.evaluate(d, result->gradient ? &gradient : nullptr);
+    vqc::UhfJk jk;
+    if (unrestricted) {
+      jk = p.evaluate_unrestricted(d, db, result->gradient ? &gradient : nullptr);
+    } else {
+      vqc::JkResult r = p.evaluate(d, result->gradient ? &gradient : nullptr);
+      jk.J = std::move(r.J);
+      jk.Ka = std::move(r.K);
+    }
     const auto& o = p.options();
     const std::vector<double> h = p.hcore();
-    std::vector<double> fock(matrix);
+    std::vector<double> fock(matrix), fock_b(unrestricted ? matrix : 0);
     double e1 = 0.0, ej = 0.0, ek = 0.0, cj = 0.0, ck = 0.0;
     // Compensated sums as in fock_build.cpp:460-491.
     auto kahan = [](double& sum, double& comp, double v) {
@@ -367,23 +379,34 @@ generativeqc_status generativeqc_fock_plan_evaluate(generativeqc_fock_plan* plan
     };
     double c1 = 0.0;
     for (uint64_t i = 0; i < matrix; ++i) {
-      fock[i] = h[i] + (o.coulomb ? o.cj * jk.J[i] : 0.0) + (o.exchange ? o.ck * jk.K[i] : 0.0);
-      kahan(e1, c1, d[i] * h[i]);
-      if (o.coulomb) kahan(ej, cj, 0.5 * d[i] * o.cj * jk.J[i]);
-      if (o.exchange) kahan(ek, ck, 0.5 * d[i] * o.ck * jk.K[i]);
+      const double total = d[i] + (unrestricted ? db[i] : 0.0);
+      const double jpart = o.coulomb ? o.cj * jk.J[i] : 0.0;
+      fock[i] = h[i] + jpart + (o.exchange ? o.ck * jk.Ka[i] : 0.0);
+      if (unrestricted) fock_b[i] = h[i] + jpart + (o.exchange ? o.ck * jk.Kb[i] : 0.0);
+      kahan(e1, c1, total * h[i]);
+      if (o.coulomb) kahan(ej, cj, 0.5 * total * o.cj * jk.J[i]);
+      if (o.exchange) {
+        kahan(ek, ck, 0.5 * d[i] * o.ck * jk.Ka[i]);
+        if (unrestricted) kahan(ek, ck, 0.5 * db[i] * o.ck * jk.Kb[i]);
+      }
     }
     const double e2 = ej + ek;
     if (!std::isfinite(e1) || !std::isfinite(e2)) throw std::runtime_error("nonfinite Fock energy");
     finite(jk.J);
-    finite(jk.K);
+    finite(jk.Ka);
+    finite(jk.Kb);
     finite(fock);
+    finite(fock_b);
     if (result->coulomb && o.coulomb) std::copy(jk.J.begin(), jk.J.end(), result->coulomb);
-    if (result->exchange_alpha && o.exchange) std::copy(jk.K.begin(), jk.K.end(), result->exchange_alpha);
+    if (result->exchange_alpha && o.exchange) std::copy(jk.Ka.begin(), jk.Ka.end(), result->exchange_alpha);
+    if (unrestricted && result->exchange_beta && o.exchange)
+      std::copy(jk.Kb.begin(), jk.Kb.end(), result->exchange_beta);
     if (result->gradient) {
       finite(gradient);
       std::copy(gradient.begin(), gradient.end(), result->gradient);
     }
     if (result->fock_alpha) std::copy(fock.begin(), fock.end(), result->fock_alpha);
+    if (unrestricted && result->fock_beta) std::copy(fock_b.begin(), fock_b.end(), result->fock_beta);
     result->energy_one_electron = e1;
     result->energy_two_electron = e2;
     result->nuclear_repulsion = p.nuclear_repulsion();
@@ -402,7 +425,8 @@ generativeqc_status generativeqc_fock_plan_solve(generativeqc_fock_plan* plan,
   plan->detail.clear();
   try {
     auto& p = *plan->plan;
-    const uint64_t count = static_cast<uint64_t>(p.nbf()) * p.nbf();
+    const bool unrestricted = plan->resolved.spin == GENERATIVEQC_FOCK_UNRESTRICTED;
+    const uint64_t count = static_cast<uint64_t>(p.nbf()) * p.nbf() * (unrestricted ? 2 : 1);
     require(output->density ? output->density_count == count : output->density_count == 0,
             "SCF density output shape mismatch");
     const uint64_t coordinates = static_cast<uint64_t>(p.system().atoms.size()) * 3;
@@ -430,7 +454,9 @@ generativeqc_status generativeqc_fock_plan_solve(generativeqc_fock_plan* plan,
       seed.assign(initial_density, initial_density + count);
       finite(seed);
     }
-    const vqc::ScfOutcome r = p.solve(opts, initial_density ? &seed : nullptr, output->forces != nullptr);
+    const vqc::ScfOutcome r =
+        unrestricted ? p.solve_unrestricted(opts, initial_density ? &seed : nullptr, output->forces != nullptr)
+                     : p.solve(opts, initial_density ? &seed : nullptr, output->forces != nullptr);
     plan->solve_stats[0] = r.iterations;
     plan->solve_stats[1] = r.reduced_iterations;
     plan->solve_stats[2] = r.purified_iterations;

@@ -49,10 +49,10 @@ __global__ void rank1_kernel(const double* __restrict__ c, int na, double alpha,
   W[e] += alpha * c[e / na] * c[e % na];
 }
 
-// W_A[packed(mu, nu)][P] = tmp[nu - nu0][mu][P] + cj D[mu][nu] c[P], mu >= nu.
+// W_A[packed(mu, nu)][P] (+)= tmp[nu - nu0][mu][P] + cj D[mu][nu] c[P], mu >= nu.
 __global__ void pack_weights_kernel(const double* __restrict__ tmp, int nu0, int nb, int n, int ldq,
                                     const double* __restrict__ D, const double* __restrict__ c, double cj,
-                                    double* __restrict__ WA) {
+                                    int accumulate, double* __restrict__ WA) {
   const std::size_t e = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const std::size_t total = static_cast<std::size_t>(nb) * n * ldq;
   if (e >= total) return;
@@ -61,7 +61,8 @@ __global__ void pack_weights_kernel(const double* __restrict__ tmp, int nu0, int
   const int mu = static_cast<int>(rest % n), nu = nu0 + static_cast<int>(rest / n);
   if (mu < nu) return;
   const std::size_t p = static_cast<std::size_t>(mu) * (mu + 1) / 2 + nu;
-  WA[p * ldq + P] = tmp[e] + cj * D[static_cast<std::size_t>(mu) * n + nu] * c[P];
+  const double w = tmp[e] + cj * D[static_cast<std::size_t>(mu) * n + nu] * c[P];
+  WA[p * ldq + P] = accumulate ? WA[p * ldq + P] + w : w;
 }
 
 // Row-major F (n x ldc): out[mu][i] = F[mu][i] * e[i] for i < r, else 0.
@@ -79,18 +80,32 @@ unsigned blocks_for(std::size_t count) { return static_cast<unsigned>((count + 2
 
 void DfPlan::df_two_electron_gradient(int r, int r_pos, double* grad) {
   if (!cholesky_) throw NotImplemented("the analytic DF gradient needs a full-rank (Cholesky-factored) metric");
+  DeviceArray<double> WA(npair_ * ldq_), WM(static_cast<std::size_t>(naux_) * naux_);
+  df_gradient_weights(r, r_pos, opts_.coulomb ? opts_.cj : 0.0, opts_.exchange ? opts_.ck : 0.0, false, WA.data(),
+                      WM.data());
+  contract_df_gradient(WA, WM, grad);
+}
+
+void DfPlan::contract_df_gradient(DeviceArray<double>& WA, DeviceArray<double>& WM, double* grad) {
+  compute_three_center_gradient(orb_, ob_, aux_, al_, boys_.data(), WA.data(), ldq_, opts_.screening * 1e-10, grad,
+                                stream_);
+  WA.release();
+  compute_metric_gradient(aux_, al_, boys_.data(), WM.data(), naux_, static_cast<int>(orb_.atoms.size()), grad,
+                          stream_);
+}
+
+void DfPlan::df_gradient_weights(int r, int r_pos, double cj, double ck, bool accumulate, double* WA, double* WM) {
+  if (!cholesky_) throw NotImplemented("the analytic DF gradient needs a full-rank (Cholesky-factored) metric");
   const int n = n_, na = naux_, lq = ldq_;
-  const double cj = opts_.coulomb ? opts_.cj : 0.0;
-  const double ck = opts_.exchange ? opts_.ck : 0.0;
   const double one = 1.0, zero = 0.0;
+  const double wm_beta = accumulate ? 1.0 : 0.0;
+  const int acc = accumulate ? 1 : 0;
   // c = L^-T rho_B (padding rows stay zero).
   DeviceArray<double> c(lq);
   copy_device(c.data(), rho_.data(), lq, stream_);
   VQC_BLAS(rocblas_dtrsm(blas_, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
                          rocblas_diagonal_non_unit, na, 1, &one, L_.data(), na, c.data(), lq));
-  DeviceArray<double> WM(static_cast<std::size_t>(na) * na);
-  VQC_HIP(hipMemsetAsync(WM.data(), 0, WM.bytes(), stream_));
-  DeviceArray<double> WA(npair_ * lq);
+  if (!accumulate) VQC_HIP(hipMemsetAsync(WM, 0, static_cast<std::size_t>(na) * na * sizeof(double), stream_));
   const std::size_t rl = static_cast<std::size_t>(r) * lq;
   if (ck != 0.0 && r > 0) {
     // T[j][i][Q] = sum_mu F[mu][j] U[mu][i][Q], then Y = L^-T T over Q.
@@ -108,11 +123,11 @@ void DfPlan::df_two_electron_gradient(int r, int r_pos, double* grad) {
                          Ys.data());
       VQC_LAUNCH("signed_pairs_kernel");
       VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_transpose, na, na, r * r, &wk, Y.data(),
-                             lq, Ys.data(), lq, &zero, WM.data(), na));
+                             lq, Ys.data(), lq, &wm_beta, WM, na));
       sync();
     } else {
       VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_transpose, na, na, r * r, &wk, Y.data(),
-                             lq, Y.data(), lq, &zero, WM.data(), na));
+                             lq, Y.data(), lq, &wm_beta, WM, na));
     }
     // V[nu][i][P] = sum_j s_j F[nu][j] Y[j][i][P] (into the U buffer, no longer needed).
     DeviceArray<double> FS(static_cast<std::size_t>(n) * ldc_);
@@ -133,7 +148,7 @@ void DfPlan::df_two_electron_gradient(int r, int r_pos, double* grad) {
                                              FS.data(), ldc_, 0, &zero, tmp.data(), lq,
                                              static_cast<rocblas_stride>(per_nu), cnt));
       hipLaunchKernelGGL(pack_weights_kernel, dim3(blocks_for(per_nu * cnt)), dim3(256), 0, stream_, tmp.data(), nu0,
-                         cnt, n, lq, D_.data(), c.data(), cj, WA.data());
+                         cnt, n, lq, D_.data(), c.data(), cj, acc, WA);
       VQC_LAUNCH("pack_weights_kernel");
     }
     sync();
@@ -145,21 +160,17 @@ void DfPlan::df_two_electron_gradient(int r, int r_pos, double* grad) {
     for (int nu0 = 0; nu0 < n; nu0 += nb) {
       const int cnt = std::min(nb, n - nu0);
       hipLaunchKernelGGL(pack_weights_kernel, dim3(blocks_for(static_cast<std::size_t>(n) * lq * cnt)), dim3(256), 0,
-                         stream_, tmp.data(), nu0, cnt, n, lq, D_.data(), c.data(), cj, WA.data());
+                         stream_, tmp.data(), nu0, cnt, n, lq, D_.data(), c.data(), cj, acc, WA);
       VQC_LAUNCH("pack_weights_kernel");
     }
     sync();
   }
   if (cj != 0.0) {
     hipLaunchKernelGGL(rank1_kernel, dim3(blocks_for(static_cast<std::size_t>(na) * na)), dim3(256), 0, stream_,
-                       c.data(), na, -0.5 * cj, WM.data());
+                       c.data(), na, -0.5 * cj, WM);
     VQC_LAUNCH("rank1_kernel");
   }
   sync();
-  compute_three_center_gradient(orb_, ob_, aux_, al_, boys_.data(), WA.data(), lq, opts_.screening * 1e-10, grad,
-                                stream_);
-  WA.release();
-  compute_metric_gradient(aux_, al_, boys_.data(), WM.data(), na, static_cast<int>(orb_.atoms.size()), grad, stream_);
 }
 
 void DfPlan::one_electron_gradient(const double* D, const double* eps, int nocc, double* grad) {

@@ -43,6 +43,11 @@ struct JkResult {
   std::vector<double> J, K;
 };
 
+// Unrestricted raw J (of the total density) and K of each spin density.
+struct UhfJk {
+  std::vector<double> J, Ka, Kb;
+};
+
 class DfPlan {
  public:
   // With stream/blas non-null the plan runs on that stream and rocBLAS handle
@@ -80,6 +85,15 @@ class DfPlan {
   // Timing harness (see vibeqc_fock_plan_benchmark_occupied).
   std::array<double, 5> benchmark_occupied(const std::vector<double>& occupied, int nocc, unsigned repeat);
 
+  // Unrestricted J/K for spin densities Da, Db (unit occupation, host in/out).
+  // With `gradient` non-null the builds run in FP64 and *gradient receives
+  // dE2/dR of E2 = 1/2 sum (Da + Db) cJ J + 1/2 sum_s D_s cK K_s.
+  UhfJk evaluate_unrestricted(const std::vector<double>& alpha, const std::vector<double>& beta,
+                              std::vector<double>* gradient = nullptr);
+  // UHF SCF (spin occupations from the system's electrons and multiplicity);
+  // initial_density and the outcome's density are [alpha; beta] (2 n^2).
+  ScfOutcome solve_unrestricted(const ScfOptions& options, const std::vector<double>* initial_density,
+                                bool forces = false);
   // With forces, the outcome carries -dE/dR of the converged energy.
   ScfOutcome solve(const ScfOptions& options, const std::vector<double>* initial_density, bool forces = false);
   // -dE/dR (3 natom) at the density the last successful solve() converged to;
@@ -101,6 +115,19 @@ class DfPlan {
   // of an FP64 build of D_ with the factor in c64_ (rho_, FP64 U_); it
   // overwrites U_.  All add into grad (device, 3 natom).
   void df_two_electron_gradient(int r, int r_pos, double* grad);
+  // Response weights of 1/2 sum D (cj J + ck K) for the state of the last
+  // FP64 build (D_, rho_, factor in c64_, U_), written or added into
+  // WA (npair x ldq) and WM (naux x naux).
+  void df_gradient_weights(int r, int r_pos, double cj, double ck, bool accumulate, double* WA, double* WM);
+  // grad += sum WA d(mu nu|P)/dR + sum WM d(P|Q)/dR; releases WA.
+  void contract_df_gradient(DeviceArray<double>& WA, DeviceArray<double>& WM, double* grad);
+  // J of D alone (rho_ and J_).
+  void coulomb_only(Precision p, const double* D);
+  // UHF two-electron gradient of device spin densities; with orbitals
+  // non-null, orbitals[s] (column-major n x n) holds D_s's factor in its
+  // first occupied[s] columns, else D_s is eigen-decomposed.
+  void unrestricted_two_electron_gradient(const double* Da, const double* Db, const double* const* orbitals,
+                                          const int* occupied, double* grad);
   void one_electron_gradient(const double* D, const double* eps, int nocc, double* grad);
   std::vector<double> nuclear_repulsion_gradient() const;
 

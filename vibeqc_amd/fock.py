@@ -292,7 +292,8 @@ class FockPlan:
 
     # -- operations ----------------------------------------------------------
     def evaluate(self, density, *, derivative: bool = False) -> FockEvaluation:
-        """Raw J/K, Fock (with Hcore) and energies at a fixed symmetric density.
+        """Raw J/K, Fock (with Hcore) and energies at a fixed symmetric density:
+        [AO, AO] for restricted spin, [2, AO, AO] (alpha, beta) for unrestricted.
 
         With ``derivative=True`` (plan created with derivative_order 1) the
         result's ``gradient`` holds the fixed-density two-electron energy
@@ -300,27 +301,35 @@ class FockPlan:
         if type(derivative) is not bool:
             raise TypeError("derivative must be bool")
         n = self.nao
+        separate = self.spec.spin == "unrestricted"
         d = np.array(density, dtype=np.float64, order="C", copy=True)
-        if d.shape != (n, n) or not np.isfinite(d).all():
-            raise ValueError("Fock density shape/finiteness mismatch")
+        if d.shape != ((2, n, n) if separate else (n, n)) or not np.isfinite(d).all():
+            raise ValueError("Fock density shape/spin/finiteness mismatch")
         with self._lock:
             self._ensure_open()
             j = np.empty((n, n)) if self.spec.coulomb.present else None
-            k = np.empty((n, n)) if self.spec.exchange.present else None
-            f = np.empty((n, n))
+            k = np.empty(d.shape) if self.spec.exchange.present else None
+            f = np.empty(d.shape)
             g = np.empty(self.basis.natom * 3) if derivative else None
             out = _native.descriptor(
                 _native.FockResultC,
                 matrix_count=n * n,
                 gradient_count=0 if g is None else g.size,
                 coulomb=_native.pointer(j),
-                exchange_alpha=_native.pointer(k),
-                exchange_beta=None,
-                fock_alpha=_native.pointer(f),
-                fock_beta=None,
+                exchange_alpha=_native.pointer(k[0] if separate and k is not None else k),
+                exchange_beta=_native.pointer(k[1] if separate and k is not None else None),
+                fock_alpha=_native.pointer(f[0] if separate else f),
+                fock_beta=_native.pointer(f[1] if separate else None),
                 gradient=_native.pointer(g),
             )
-            status = self._lib.generativeqc_fock_plan_evaluate(self._handle, _native.pointer(d), n * n, None, 0, ct.byref(out))
+            status = self._lib.generativeqc_fock_plan_evaluate(
+                self._handle,
+                _native.pointer(d[0] if separate else d),
+                n * n,
+                _native.pointer(d[1] if separate else None),
+                n * n if separate else 0,
+                ct.byref(out),
+            )
             self._check(status)
             return FockEvaluation(
                 out.energy_one_electron,
@@ -373,19 +382,22 @@ class FockPlan:
         energy_tolerance: float = 1e-10,
         density_tolerance: float = 1e-8,
     ) -> FockScfResult:
-        """Device-resident RHF SCF for the declared J/K energy (fock.py:674-776).
+        """Device-resident RHF (restricted spec) or UHF (unrestricted spec) SCF
+        for the declared J/K energy (fock.py:674-776); UHF densities are
+        [2, AO, AO] (alpha, beta).
 
         ``compute_forces`` returns the complete forces -dE/dR, (natom, 3),
         Hartree/bohr, at the converged density."""
         if type(compute_forces) is not bool:
             raise TypeError("compute_forces must be bool")
         n = self.nao
+        shape = (2, n, n) if self.spec.spin == "unrestricted" else (n, n)
         seed = None
         if initial_density is not None:
             seed = np.array(initial_density, dtype=np.float64, order="C", copy=True)
-            if seed.shape != (n, n):
-                raise ValueError("initial density shape mismatch")
-        density = np.empty((n, n))
+            if seed.shape != shape or not np.isfinite(seed).all():
+                raise ValueError("initial density shape/spin/finiteness mismatch")
+        density = np.empty(shape)
         controls = _native.descriptor(
             _native.ScfControlsC,
             max_iterations=max_iterations,
@@ -398,13 +410,13 @@ class FockPlan:
             _native.ScfResultC,
             density=_native.pointer(density),
             forces=_native.pointer(forces),
-            density_count=n * n,
+            density_count=density.size,
             force_count=0 if forces is None else forces.size,
         )
         with self._lock:
             self._ensure_open()
             status = self._lib.generativeqc_fock_plan_solve(
-                self._handle, ct.byref(controls), _native.pointer(seed), 0 if seed is None else n * n, ct.byref(out)
+                self._handle, ct.byref(controls), _native.pointer(seed), 0 if seed is None else seed.size, ct.byref(out)
             )
             self._check(status)
         return FockScfResult(
