@@ -138,6 +138,39 @@ GENERATIVEQC_API generativeqc_status generativeqc_batch_set_warm_start_updates(g
 GENERATIVEQC_API generativeqc_status generativeqc_batch_get_last_fock_builds(const generativeqc_batch* batch,
                                                                              uint32_t index, uint64_t* builds);
 
+/* Reference: generativeqc.h:1497-1509: a retained converged density (unit
+ * occupation doubled: the restricted D), the coordinates it converged at and
+ * its source diagnostics. */
+typedef struct generativeqc_hf_warm_state {
+  uint32_t struct_size;
+  uint32_t abi_version;
+  double* density;
+  uint64_t density_count;
+  double* coordinates;
+  uint64_t coordinate_count;
+  double energy;
+  double energy_change;
+  double density_rms;
+  int32_t iterations;
+  int32_t present;
+} generativeqc_hf_warm_state;
+
+/* Reference: generativeqc.h:1511-1514.  With both buffers NULL the call fills
+ * the counts, diagnostics and `present`; otherwise the counts must equal the
+ * retained sizes and both arrays are copied.  No seed: present = 0, counts 0. */
+GENERATIVEQC_API generativeqc_status generativeqc_batch_get_hf_warm_state(const generativeqc_batch* batch,
+                                                                          uint32_t index,
+                                                                          generativeqc_hf_warm_state* state);
+/* Reference: generativeqc.h:1516-1525.  `count` must equal the system count;
+ * present = 0 keeps that item's seed.  Every present state is validated
+ * (shape, finiteness, symmetry) before any is applied; warm starts must be
+ * enabled.  The caller checks scientific identity before calling. */
+GENERATIVEQC_API generativeqc_status generativeqc_batch_restore_hf_warm_states(generativeqc_batch* batch,
+                                                                               const generativeqc_hf_warm_state* states,
+                                                                               uint32_t count);
+/* Reference: generativeqc.h:1527-1529: discard every retained seed. */
+GENERATIVEQC_API generativeqc_status generativeqc_batch_clear_warm_starts(generativeqc_batch* batch);
+
 /* Extension (no counterpart in the reference): one auxiliary system per batch
  * member, so members need not share an atom topology.  auxiliary_systems may
  * be NULL to fit with each orbital basis itself. */

@@ -264,6 +264,22 @@ class BatchItemResultC(ct.Structure):
     ]
 
 
+class HfWarmState(ct.Structure):
+    _fields_ = [
+        ("struct_size", ct.c_uint32),
+        ("abi_version", ct.c_uint32),
+        ("density", _DP),
+        ("density_count", ct.c_uint64),
+        ("coordinates", _DP),
+        ("coordinate_count", ct.c_uint64),
+        ("energy", ct.c_double),
+        ("energy_change", ct.c_double),
+        ("density_rms", ct.c_double),
+        ("iterations", ct.c_int32),
+        ("present", ct.c_int32),
+    ]
+
+
 def descriptor(kind, **values):
     """A versioned descriptor with struct_size / abi_version filled in."""
     return kind(struct_size=ct.sizeof(kind), abi_version=ABI_VERSION, **values)
@@ -328,6 +344,9 @@ EXPORTS = {
     "generativeqc_batch_set_warm_start_updates": (ct.c_int32, [ct.c_void_p, ct.c_int32]),
     "generativeqc_batch_get_last_fock_builds": (ct.c_int32, [ct.c_void_p, ct.c_uint32, ct.POINTER(ct.c_uint64)]),
     "vibeqc_batch_bucket": (ct.c_int32, [ct.c_void_p, ct.c_uint32, ct.POINTER(ct.c_uint32)]),
+    "generativeqc_batch_get_hf_warm_state": (ct.c_int32, [ct.c_void_p, ct.c_uint32, ct.POINTER(HfWarmState)]),
+    "generativeqc_batch_restore_hf_warm_states": (ct.c_int32, [ct.c_void_p, ct.POINTER(HfWarmState), ct.c_uint32]),
+    "generativeqc_batch_clear_warm_starts": (ct.c_int32, [ct.c_void_p]),
 }
 
 

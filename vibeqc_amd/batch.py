@@ -50,6 +50,9 @@ class BatchItemResult:
     warm_start_used: bool
     warm_start_fallback: bool
     fock_builds: int | None = None
+    # "cold", "cold_fallback", "in_process_warm" or "persistent_restart"
+    # (a seed restored from a checkpoint), as in the reference's batch items.
+    restart_origin: str = "cold"
 
     @property
     def succeeded(self) -> bool:
@@ -95,6 +98,12 @@ class PreparedBatch:
             raise ValueError("charges/multiplicities must match the systems")
         self._lock = threading.RLock()
         self._natoms = [len(s) for s in systems]
+        self._calculator = calculator
+        self._systems = systems
+        self._charges = [int(q) for q in charges]
+        self._multiplicities = [int(m) for m in multiplicities]
+        self._last_status = [-1] * len(systems)
+        self._restart_indices: set = set()
         self._ctx = NativeContext(calculator.device_id)
         lib = self._lib = self._ctx.lib
         handles = []
@@ -178,6 +187,15 @@ class PreparedBatch:
                 r = results[i]
                 self._lib.generativeqc_batch_get_last_fock_builds(self._handle, i, ct.byref(builds))
                 ok = r.status == 0
+                self._last_status[i] = r.status
+                if r.warm_start_fallback:
+                    origin = "cold_fallback"
+                elif r.warm_start_used:
+                    origin = "persistent_restart" if i in self._restart_indices else "in_process_warm"
+                else:
+                    origin = "cold"
+                if ok:
+                    self._restart_indices.discard(i)  # the retained seed is now this run's
                 f = forces[i]
                 if f is not None and ok:
                     f.setflags(write=False)
@@ -197,12 +215,39 @@ class PreparedBatch:
                         warm_start_used=bool(r.warm_start_used),
                         warm_start_fallback=bool(r.warm_start_fallback),
                         fock_builds=int(builds.value),
+                        restart_origin=origin,
                     )
                 )
         result = BatchResult(tuple(items))
         if strict:
             result.raise_for_failures()
         return result
+
+    def _ensure_open(self) -> None:
+        if not getattr(self, "_handle", None):
+            raise RuntimeError("batch is closed")
+
+    def save_checkpoint(self, path, *, max_bytes: int = 256 << 20) -> dict:
+        """Atomically persist the retained converged densities with their
+        identities and source diagnostics (batch.py:1312-1322)."""
+        from .checkpoint import save_checkpoint
+
+        with self._lock:
+            return save_checkpoint(self, path, max_bytes=max_bytes)
+
+    def load_checkpoint(self, path, *, allow_warm: bool = False, strict: bool = True, max_bytes: int = 256 << 20) -> dict:
+        """Restore compatible seeds for the next execution (batch.py:1324-1343):
+        exact restarts by default, changed geometry/controls with allow_warm."""
+        from .checkpoint import load_checkpoint
+
+        with self._lock:
+            return load_checkpoint(self, path, allow_warm=allow_warm, strict=strict, max_bytes=max_bytes)
+
+    def clear_warm_starts(self) -> None:
+        with self._lock:
+            self._ensure_open()
+            _native.check(self._lib.generativeqc_batch_clear_warm_starts(self._handle), "clearing warm starts failed")
+            self._restart_indices.clear()
 
     def close(self) -> None:
         with self._lock:

@@ -79,6 +79,38 @@ BatchPlan::BatchPlan(int device, std::vector<System> systems, std::vector<System
   warm_.resize(systems_.size());
 }
 
+void BatchPlan::restore_warm_states(const std::vector<const WarmState*>& states) {
+  if (!opts_.warm_starts) throw std::invalid_argument("restoring seeds requires warm starts enabled");
+  if (states.size() != systems_.size()) throw std::invalid_argument("one warm state per batch system is required");
+  for (std::size_t i = 0; i < states.size(); ++i) {
+    const WarmState* w = states[i];
+    if (!w) continue;
+    const std::size_t n = static_cast<std::size_t>(systems_[i].nbf());
+    if (w->density.size() != n * n || w->coordinates.size() != systems_[i].atoms.size() * 3)
+      throw std::invalid_argument("warm state shape does not match the batch system");
+    double scale = 0.0;
+    for (double v : w->density) {
+      if (!std::isfinite(v)) throw std::invalid_argument("warm state density must be finite");
+      scale = std::max(scale, std::abs(v));
+    }
+    for (double v : w->coordinates)
+      if (!std::isfinite(v)) throw std::invalid_argument("warm state coordinates must be finite");
+    for (std::size_t r = 0; r < n; ++r)
+      for (std::size_t c = 0; c < r; ++c)
+        if (std::abs(w->density[r * n + c] - w->density[c * n + r]) > 1e-10 * std::max(1.0, scale))
+          throw std::invalid_argument("warm state density must be symmetric");
+    if (!std::isfinite(w->energy) || !std::isfinite(w->energy_change) || !std::isfinite(w->density_rms) ||
+        w->density_rms < 0.0 || w->iterations < 0)
+      throw std::invalid_argument("warm state diagnostics must be finite");
+  }
+  for (std::size_t i = 0; i < states.size(); ++i)
+    if (states[i]) warm_[i] = *states[i];
+}
+
+void BatchPlan::clear_warm_states() {
+  for (auto& w : warm_) w = WarmState{};
+}
+
 std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::vector<double>*>& coordinates,
                                                  const std::vector<bool>& forces) {
   const std::size_t count = systems_.size();
@@ -153,8 +185,8 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
     try {
       DfPlan& plan = *plans[k];
       const std::vector<double>* seed = nullptr;
-      if (opts_.warm_starts && warm_[i].size() == static_cast<std::size_t>(plan.nbf()) * plan.nbf()) {
-        seed = &warm_[i];
+      if (opts_.warm_starts && warm_[i].density.size() == static_cast<std::size_t>(plan.nbf()) * plan.nbf()) {
+        seed = &warm_[i].density;
         r.warm_start_used = true;
       }
       try {
@@ -202,7 +234,20 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
             AllocationScope scope(streams[k % workers]);
             r.scf.forces = plans[k]->converged_forces();
           }
-          if (opts_.warm_starts && warm_updates_) warm_[i] = r.scf.density;
+          if (opts_.warm_starts && warm_updates_) {
+            WarmState& ws = warm_[i];
+            ws.density = r.scf.density;
+            if (!coordinates.empty() && coordinates[i]) {
+              ws.coordinates = *coordinates[i];
+            } else {
+              ws.coordinates.clear();
+              for (const auto& a : systems_[i].atoms) ws.coordinates.insert(ws.coordinates.end(), a.xyz, a.xyz + 3);
+            }
+            ws.energy = r.scf.energy;
+            ws.energy_change = r.scf.energy_change;
+            ws.density_rms = r.scf.density_rms;
+            ws.iterations = static_cast<int>(r.scf.iterations);
+          }
         } catch (...) {
           r.error = std::current_exception();
         }

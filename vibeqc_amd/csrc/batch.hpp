@@ -34,6 +34,15 @@ struct BatchItemOutcome {
   bool warm_start_fallback = false;
 };
 
+// A retained converged density (the seed of the item's next execution) with
+// the coordinates it was converged at and its source diagnostics
+// (reference generativeqc.h:1497-1529).  An empty density: no seed.
+struct WarmState {
+  std::vector<double> density, coordinates;
+  double energy = 0.0, energy_change = 0.0, density_rms = 0.0;
+  int iterations = 0;
+};
+
 class BatchPlan {
  public:
   // aux[i] is the auxiliary basis of systems[i] (same atoms).
@@ -42,6 +51,14 @@ class BatchPlan {
   int bucket_of(std::size_t i) const { return bucket_[i]; }
   int bucket_count() const { return buckets_; }
   void set_warm_updates(bool enabled) { warm_updates_ = enabled; }
+  bool warm_starts() const { return opts_.warm_starts; }
+  int nbf_of(std::size_t i) const { return systems_[i].nbf(); }
+  std::size_t atoms_of(std::size_t i) const { return systems_[i].atoms.size(); }
+  const WarmState& warm_state(std::size_t i) const { return warm_[i]; }
+  // Import seeds: states[i] non-null replaces item i's seed.  Every state is
+  // validated (shape, finiteness, symmetry) before any is applied.
+  void restore_warm_states(const std::vector<const WarmState*>& states);
+  void clear_warm_states();
   // coordinates[i]: replacement flat xyz (Bohr) for system i, or null.
   // forces[i]: whether item i reports forces.
   std::vector<BatchItemOutcome> execute(const std::vector<const std::vector<double>*>& coordinates,
@@ -54,7 +71,7 @@ class BatchPlan {
   std::vector<int> bucket_;
   int buckets_ = 0;
   std::vector<std::size_t> order_;  // most expensive first
-  std::vector<std::vector<double>> warm_;
+  std::vector<WarmState> warm_;
   bool warm_updates_ = true;
 };
 

@@ -645,6 +645,66 @@ generativeqc_status generativeqc_batch_get_last_fock_builds(const generativeqc_b
   return GENERATIVEQC_STATUS_SUCCESS;
 }
 
+generativeqc_status generativeqc_batch_get_hf_warm_state(const generativeqc_batch* batch, uint32_t index,
+                                                         generativeqc_hf_warm_state* state) {
+  if (!batch || !state || index >= batch->plan->size()) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  if (!valid_descriptor(state)) return GENERATIVEQC_STATUS_ABI_MISMATCH;
+  const vqc::WarmState& w = batch->plan->warm_state(index);
+  const bool present = !w.density.empty();
+  if (state->density || state->coordinates) {
+    if (!present || !state->density || !state->coordinates || state->density_count != w.density.size() ||
+        state->coordinate_count != w.coordinates.size())
+      return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+    std::copy(w.density.begin(), w.density.end(), state->density);
+    std::copy(w.coordinates.begin(), w.coordinates.end(), state->coordinates);
+  } else {
+    state->density_count = present ? w.density.size() : 0;
+    state->coordinate_count = present ? w.coordinates.size() : 0;
+  }
+  state->energy = present ? w.energy : 0.0;
+  state->energy_change = present ? w.energy_change : 0.0;
+  state->density_rms = present ? w.density_rms : 0.0;
+  state->iterations = present ? w.iterations : 0;
+  state->present = present ? 1 : 0;
+  return GENERATIVEQC_STATUS_SUCCESS;
+}
+
+generativeqc_status generativeqc_batch_restore_hf_warm_states(generativeqc_batch* batch,
+                                                              const generativeqc_hf_warm_state* states,
+                                                              uint32_t count) {
+  if (!batch || !states) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  batch->detail.clear();
+  try {
+    require(count == batch->plan->size(), "warm state count mismatch");
+    std::vector<vqc::WarmState> owned(count);
+    std::vector<const vqc::WarmState*> view(count, nullptr);
+    for (uint32_t i = 0; i < count; ++i) {
+      if (!valid_descriptor(&states[i])) return GENERATIVEQC_STATUS_ABI_MISMATCH;
+      const auto& s = states[i];
+      require(s.present == 0 || s.present == 1, "warm state presence must be zero or one");
+      if (!s.present) continue;
+      require(s.density && s.coordinates, "present warm states need both arrays");
+      owned[i].density.assign(s.density, s.density + s.density_count);
+      owned[i].coordinates.assign(s.coordinates, s.coordinates + s.coordinate_count);
+      owned[i].energy = s.energy;
+      owned[i].energy_change = s.energy_change;
+      owned[i].density_rms = s.density_rms;
+      owned[i].iterations = s.iterations;
+      view[i] = &owned[i];
+    }
+    batch->plan->restore_warm_states(view);
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(&batch->detail);
+  }
+}
+
+generativeqc_status generativeqc_batch_clear_warm_starts(generativeqc_batch* batch) {
+  if (!batch) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  batch->plan->clear_warm_states();
+  return GENERATIVEQC_STATUS_SUCCESS;
+}
+
 generativeqc_status vibeqc_batch_bucket(const generativeqc_batch* batch, uint32_t index, uint32_t* bucket_id) {
   if (!batch || !bucket_id || index >= batch->plan->size()) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
   *bucket_id = static_cast<uint32_t>(batch->plan->bucket_of(index));
