@@ -6,10 +6,12 @@
  * points for the RHF density-fitted path (include/generativeqc/generativeqc.h:
  * method descriptor :723-769, batch flags :376-397, input/result descriptors
  * :1158-1183, prepare/destroy/count :1431-1441, execute :1574-1586, warm-start
- * control :1563-1572, Fock-build count :1292-1297).  Only
- * GENERATIVEQC_METHOD_RHF with density fitting (CUDA or AUTO mode, executed
- * on the HIP device) is implemented; other methods, KS options and initial
- * guess options return NOT_IMPLEMENTED.  precision_mode AUTO runs BF16X3 Fock
+ * control :1563-1572, Fock-build count :1292-1297; method ids from
+ * generated_method_ids.h:11-12).  GENERATIVEQC_METHOD_RHF and
+ * GENERATIVEQC_METHOD_UHF with density fitting (CUDA or AUTO mode, executed
+ * on the HIP device) are implemented; other methods, KS options and initial
+ * guess options return NOT_IMPLEMENTED.  A UHF batch retains [alpha; beta]
+ * densities (2 nbf^2 values) as warm states.  precision_mode AUTO runs BF16X3 Fock
  * builds followed by an FP64 refinement to the requested tolerances.
  */
 #ifndef GENERATIVEQC_BATCH_H
@@ -22,7 +24,7 @@ extern "C" {
 #endif
 
 typedef int32_t generativeqc_method;
-enum { GENERATIVEQC_METHOD_RHF = 1 };
+enum { GENERATIVEQC_METHOD_RHF = 1, GENERATIVEQC_METHOD_UHF = 2 };
 
 typedef int32_t generativeqc_density_fitting_mode;
 enum {

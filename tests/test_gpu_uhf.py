@@ -151,3 +151,40 @@ def test_mixed_precision_uhf_and_seeded_restart():
         again = plan.solve(initial_density=np.asarray(e64.density))
     assert abs(again.energy - e64.energy) < 1e-9
     assert again.iterations <= 3
+
+
+def test_uhf_batch_matches_single_plans_and_checkpoints(tmp_path):
+    from vibeqc_amd import Calculator
+    from vibeqc_amd.basis import Atom
+    from vibeqc_amd.checkpoint import inspect_checkpoint
+    from vibeqc_amd.geometry import water_cluster
+
+    systems = [
+        OH,
+        tuple(water_cluster(1, 3.0, seed=5)),
+        (Atom(1, (0.0, 0.0, 0.0)),),
+        (Atom(8, (0.0, 0.0, 0.0)), Atom(1, (0.0, 0.2, 1.9))),
+    ]
+    mult = [2, 1, 2, 2]
+    calc = Calculator(method="uhf")
+    batch = calc.batch_singlepoint(systems, multiplicities=mult)
+    assert batch.succeeded
+    for item, atoms, m in zip(batch.items, systems, mult):
+        single = calc.singlepoint(atoms, multiplicity=m)
+        print(f"item {item.index}: E={item.energy:.10f} dE={item.energy - single.energy:.2e} it={item.iterations}")
+        assert abs(item.energy - single.energy) < 1e-9
+        assert np.abs(np.asarray(item.forces) - np.asarray(single.forces)).max() < 1e-8
+    path = tmp_path / "uhf.vqcp"
+    with calc.prepare_batch(systems, multiplicities=mult) as prepared:
+        first = prepared.execute(properties=("energy",), strict=True)
+        prepared.save_checkpoint(path)
+    doc = inspect_checkpoint(path)
+    n = [it["seed"]["nbf"] for it in doc["items"]]
+    assert [b["shape"] for b in doc["blobs"] if b["name"].startswith("density")] == [[2, k, k] for k in n]
+    with calc.prepare_batch(systems, multiplicities=mult) as prepared:
+        prepared.load_checkpoint(path)
+        restored = prepared.execute(properties=("energy",), strict=True)
+    for a, b in zip(first.items, restored.items):
+        assert b.restart_origin == "persistent_restart"
+        assert abs(a.energy - b.energy) < 1e-9
+        assert b.iterations <= 3

@@ -199,6 +199,7 @@ class FockDiagnosticC(ct.Structure):
 
 
 METHOD_RHF = 1
+METHOD_UHF = 2
 DENSITY_FITTING_CUDA = 2
 PRECISION_MODE = {"fp64": 0, "auto": 1}
 BATCH_WARM_STARTS = 1

@@ -1,8 +1,7 @@
 """HF ``Calculator`` entry point (host mirror of python/generativeqc/calculator.py:146-200).
 
 The density-fitted Hartree-Fock paths are provided: ``method="rhf"``
-(closed shell) and ``method="uhf"`` (any multiplicity; ragged batches are
-RHF only), with ``density_fitting`` in ("hip", "cuda", "auto", True).  Coordinates are in
+(closed shell) and ``method="uhf"`` (any multiplicity; also in ragged batches), with ``density_fitting`` in ("hip", "cuda", "auto", True).  Coordinates are in
 Bohr; results carry the energy in Hartree and convergence diagnostics.
 ``precision`` selects the J/K arithmetic: "fp64" (the reference's DF
 precision), "fp32", "bf16x3", or "mixed" (= bf16x3 iterations followed by FP64
@@ -120,13 +119,11 @@ class Calculator:
         """generativeqc_method_descriptor of this calculator (generativeqc.h:723-769)."""
         from . import _native
 
-        if self.method != "rhf":
-            raise NotImplementedError("ragged batches run restricted (RHF) systems")
         if self.precision not in ("fp64", "mixed", "bf16x3"):
             raise NotImplementedError("batches run in fp64 or mixed (bf16x3 + FP64 refinement) precision")
         return _native.descriptor(
             _native.MethodDescriptorC,
-            method=_native.METHOD_RHF,
+            method=_native.METHOD_UHF if self.method == "uhf" else _native.METHOD_RHF,
             max_iterations=self.max_iterations,
             diis_history=self.diis_history,
             energy_tolerance=self.energy_tolerance,

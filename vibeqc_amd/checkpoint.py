@@ -10,7 +10,7 @@ manifest (schema ``generativeqc.hf_checkpoint``, version 1) lists one item per
 batch slot in input order with its ordered nuclei, scientific model identity,
 basis metadata, solver controls, source backend, last status and, when a
 converged seed is retained, the seed's diagnostics and blob names
-(``density_<i>`` of shape [1, n, n], ``coordinates_<i>`` of shape [natom, 3]).
+(``density_<i>`` of shape [spins, n, n], ``coordinates_<i>`` of shape [natom, 3]).
 
 Identities follow the reference's definitions where they are formulas over
 the inputs (the geometry hash over ordered nuclei and hex-spelled
@@ -342,7 +342,8 @@ def _warm_state(batch, index: int):
 
 def build_checkpoint(calculator, systems, charges, multiplicities, statuses, seeds, *, max_bytes: int = 256 << 20) -> bytes:
     """The bytes of a checkpoint file.  seeds[i] is None or a dict with the
-    retained restricted density (n x n), the coordinates it converged at
+    retained density (n x n restricted, 2 x n x n alpha/beta for UHF), the
+    coordinates it converged at
     (natom x 3, Bohr) and its energy, energy_change, density_rms, iterations."""
     from .basis import Atom
 
@@ -368,6 +369,7 @@ def build_checkpoint(calculator, systems, charges, multiplicities, statuses, see
         }
         seed = seeds[i]
         if seed is not None:
+            spins = 2 if calculator.method == "uhf" else 1
             density = np.asarray(seed["density"], dtype=np.float64)
             coords = np.asarray(seed["coordinates"], dtype=np.float64).reshape(-1, 3)
             n = density.shape[-1]
@@ -384,7 +386,7 @@ def build_checkpoint(calculator, systems, charges, multiplicities, statuses, see
                 "iterations": int(seed["iterations"]),
                 "converged": True,
             }
-            for name, array in (("density", density.reshape(1, n, n)), ("coordinates", coords)):
+            for name, array in (("density", density.reshape(spins, n, n)), ("coordinates", coords)):
                 a = np.ascontiguousarray(array, dtype="<f8")
                 if not np.isfinite(a).all():
                     raise CheckpointError("nonfinite retained state")
@@ -430,10 +432,11 @@ def save_checkpoint(batch, path, *, max_bytes: int = 256 << 20) -> dict:
             seeds.append(None)
             continue
         density, coords, st = state
-        n = math.isqrt(density.size)
+        spins = 2 if batch._calculator.method == "uhf" else 1
+        n = math.isqrt(density.size // spins)
         seeds.append(
             {
-                "density": density.reshape(n, n),
+                "density": density.reshape(spins, n, n) if spins == 2 else density.reshape(n, n),
                 "coordinates": coords,
                 "energy": st.energy,
                 "energy_change": st.energy_change,

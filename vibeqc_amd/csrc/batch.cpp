@@ -59,10 +59,13 @@ BatchPlan::BatchPlan(int device, std::vector<System> systems, std::vector<System
     if (aux_[i].atoms.size() != systems_[i].atoms.size())
       throw std::invalid_argument("auxiliary basis atoms do not match the batch system");
   // Shape buckets in order of first appearance.
-  std::map<std::tuple<std::size_t, std::size_t, int>, int> ids;
+  std::map<std::tuple<std::size_t, std::size_t, int, int>, int> ids;
   bucket_.resize(systems_.size());
   for (std::size_t i = 0; i < systems_.size(); ++i) {
-    const auto key = std::make_tuple(systems_[i].nbf(), aux_[i].nbf(), systems_[i].electrons / 2);
+    // (nbf, naux, n_alpha, n_beta); a restricted member has n_alpha = n_beta.
+    const int excess = static_cast<int>(systems_[i].multiplicity) - 1;
+    const int na = (systems_[i].electrons + excess) / 2, nb = (systems_[i].electrons - excess) / 2;
+    const auto key = std::make_tuple(systems_[i].nbf(), aux_[i].nbf(), na, nb);
     auto it = ids.find(key);
     if (it == ids.end()) it = ids.emplace(key, static_cast<int>(ids.size())).first;
     bucket_[i] = it->second;
@@ -86,7 +89,8 @@ void BatchPlan::restore_warm_states(const std::vector<const WarmState*>& states)
     const WarmState* w = states[i];
     if (!w) continue;
     const std::size_t n = static_cast<std::size_t>(systems_[i].nbf());
-    if (w->density.size() != n * n || w->coordinates.size() != systems_[i].atoms.size() * 3)
+    const std::size_t spins = opts_.unrestricted ? 2 : 1;
+    if (w->density.size() != spins * n * n || w->coordinates.size() != systems_[i].atoms.size() * 3)
       throw std::invalid_argument("warm state shape does not match the batch system");
     double scale = 0.0;
     for (double v : w->density) {
@@ -95,10 +99,12 @@ void BatchPlan::restore_warm_states(const std::vector<const WarmState*>& states)
     }
     for (double v : w->coordinates)
       if (!std::isfinite(v)) throw std::invalid_argument("warm state coordinates must be finite");
-    for (std::size_t r = 0; r < n; ++r)
-      for (std::size_t c = 0; c < r; ++c)
-        if (std::abs(w->density[r * n + c] - w->density[c * n + r]) > 1e-10 * std::max(1.0, scale))
-          throw std::invalid_argument("warm state density must be symmetric");
+    for (std::size_t b = 0; b < spins; ++b)
+      for (std::size_t r = 0; r < n; ++r)
+        for (std::size_t c = 0; c < r; ++c)
+          if (std::abs(w->density[b * n * n + r * n + c] - w->density[b * n * n + c * n + r]) >
+              1e-10 * std::max(1.0, scale))
+            throw std::invalid_argument("warm state density must be symmetric");
     if (!std::isfinite(w->energy) || !std::isfinite(w->energy_change) || !std::isfinite(w->density_rms) ||
         w->density_rms < 0.0 || w->iterations < 0)
       throw std::invalid_argument("warm state diagnostics must be finite");
@@ -185,18 +191,22 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
     try {
       DfPlan& plan = *plans[k];
       const std::vector<double>* seed = nullptr;
-      if (opts_.warm_starts && warm_[i].density.size() == static_cast<std::size_t>(plan.nbf()) * plan.nbf()) {
+      const std::size_t spins = opts_.unrestricted ? 2 : 1;
+      if (opts_.warm_starts && warm_[i].density.size() == spins * plan.nbf() * plan.nbf()) {
         seed = &warm_[i].density;
         r.warm_start_used = true;
       }
+      auto run = [&](const std::vector<double>* s) {
+        return opts_.unrestricted ? plan.solve_unrestricted(opts_.scf, s, false) : plan.solve(opts_.scf, s, false);
+      };
       try {
-        r.scf = plan.solve(opts_.scf, seed, false);
+        r.scf = run(seed);
       } catch (const std::invalid_argument&) {
         if (!seed) throw;
         // A retained density the guards reject falls back to the core guess.
         r.warm_start_used = false;
         r.warm_start_fallback = true;
-        r.scf = plan.solve(opts_.scf, nullptr, false);
+        r.scf = run(nullptr);
       }
     } catch (...) {
       r.error = std::current_exception();

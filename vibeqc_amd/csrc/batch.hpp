@@ -23,6 +23,7 @@ struct BatchOptions {
   ScfOptions scf;
   Precision precision = Precision::FP64;
   bool warm_starts = false;
+  bool unrestricted = false;  // UHF members (GENERATIVEQC_METHOD_UHF), else RHF
   int workers = 0;  // 0: VIBEQC_BATCH_WORKERS if set, else 8 (never more than the systems)
 };
 
@@ -34,7 +35,8 @@ struct BatchItemOutcome {
   bool warm_start_fallback = false;
 };
 
-// A retained converged density (the seed of the item's next execution) with
+// A retained converged density (the seed of the item's next execution: the
+// restricted D, or [alpha; beta] for UHF members) with
 // the coordinates it was converged at and its source diagnostics
 // (reference generativeqc.h:1497-1529).  An empty density: no seed.
 struct WarmState {
@@ -52,6 +54,7 @@ class BatchPlan {
   int bucket_count() const { return buckets_; }
   void set_warm_updates(bool enabled) { warm_updates_ = enabled; }
   bool warm_starts() const { return opts_.warm_starts; }
+  bool unrestricted() const { return opts_.unrestricted; }
   int nbf_of(std::size_t i) const { return systems_[i].nbf(); }
   std::size_t atoms_of(std::size_t i) const { return systems_[i].atoms.size(); }
   const WarmState& warm_state(std::size_t i) const { return warm_[i]; }

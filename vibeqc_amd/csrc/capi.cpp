@@ -126,7 +126,8 @@ void copy_text(char* dst, std::size_t cap, const char* src) {
 
 // RHF/DF options of a method descriptor (reference generativeqc.h:723-769).
 vqc::BatchOptions batch_options(const generativeqc_method_descriptor& d, generativeqc_batch_flags flags) {
-  if (d.method != GENERATIVEQC_METHOD_RHF) throw vqc::NotImplemented("batches implement the RHF method only");
+  if (d.method != GENERATIVEQC_METHOD_RHF && d.method != GENERATIVEQC_METHOD_UHF)
+    throw vqc::NotImplemented("batches implement the RHF and UHF methods only");
   if (d.density_fitting_mode != GENERATIVEQC_DENSITY_FITTING_CUDA &&
       d.density_fitting_mode != GENERATIVEQC_DENSITY_FITTING_AUTO)
     throw vqc::NotImplemented("batches run density-fitted J/K on the HIP device only");
@@ -146,6 +147,8 @@ vqc::BatchOptions batch_options(const generativeqc_method_descriptor& d, generat
   if (d.density_tolerance > 0.0) o.scf.density_tolerance = d.density_tolerance;
   o.precision = d.precision_mode == GENERATIVEQC_PRECISION_AUTO ? vqc::Precision::BF16X3 : vqc::Precision::FP64;
   o.warm_starts = (flags & GENERATIVEQC_BATCH_ENABLE_WARM_STARTS) != 0;
+  o.unrestricted = d.method == GENERATIVEQC_METHOD_UHF;
+  o.plan.ck = o.unrestricted ? -1.0 : -0.5;  // HF exchange: -1 per spin density, -1/2 of the restricted D
   return o;
 }
 

@@ -504,6 +504,10 @@ std::vector<double> DfPlan::converged_forces() {
   VQC_HIP(hipSetDevice(device_));
   if (!converged_state_) throw std::logic_error("forces need a converged solve on this plan");
   converged_state_ = false;  // the gradient overwrites U_
+  if (converged_unrestricted_) {
+    converged_unrestricted_ = false;
+    return unrestricted_forces();
+  }
   // The last build of solve() left rho_, the FP64 U_ and the factor
   // sqrt(2) C_occ of D_ in place; eps_ holds the orbital energies of the same
   // diagonalisation.
@@ -526,6 +530,7 @@ std::vector<double> DfPlan::converged_forces() {
 ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial_density, bool forces) {
   VQC_HIP(hipSetDevice(device_));
   converged_state_ = false;
+  converged_unrestricted_ = false;
   if (orb_.multiplicity != 1 || orb_.electrons % 2 != 0)
     throw NotImplemented("only closed-shell restricted SCF is implemented");
   if (!opts_.coulomb || !opts_.exchange || opts_.cj != 1.0 || opts_.ck != -0.5)

@@ -96,8 +96,8 @@ class DfPlan {
                                 bool forces = false);
   // With forces, the outcome carries -dE/dR of the converged energy.
   ScfOutcome solve(const ScfOptions& options, const std::vector<double>* initial_density, bool forces = false);
-  // -dE/dR (3 natom) at the density the last successful solve() converged to;
-  // valid until the next operation on this plan.
+  // -dE/dR (3 natom) at the density the last successful solve() or
+  // solve_unrestricted() converged to; valid until the next operation on this plan.
   std::vector<double> converged_forces();
 
  private:
@@ -152,6 +152,12 @@ class DfPlan {
   DeviceArray<double> L_;  // Cholesky factor of (P|Q), column-major lower (Cholesky path only)
   DeviceArray<double> eps_;  // orbital energies of the last converged solve
   bool converged_state_ = false;  // D_, c64_, rho_, FP64 U_ and eps_ describe that solve
+  // The last converged unrestricted solve: [alpha; beta] orbitals (column-major
+  // n x n each), orbital energies and densities, and the spin occupations.
+  bool converged_unrestricted_ = false;
+  DeviceArray<double> uhf_orbitals_, uhf_energies_, uhf_density_;
+  int uhf_occ_[2] = {0, 0};
+  std::vector<double> unrestricted_forces();
   DeviceArray<double> S_, H_;
   DeviceArray<double> B64_;
   DeviceArray<float> B32_;

@@ -170,6 +170,7 @@ UhfJk DfPlan::evaluate_unrestricted(const std::vector<double>& alpha, const std:
 ScfOutcome DfPlan::solve_unrestricted(const ScfOptions& o, const std::vector<double>* initial_density, bool forces) {
   VQC_HIP(hipSetDevice(device_));
   converged_state_ = false;
+  converged_unrestricted_ = false;
   if (!opts_.coulomb || !opts_.exchange || opts_.cj != 1.0 || opts_.ck != -1.0)
     throw NotImplemented("unrestricted solve requires the standard Hartree-Fock J/K coefficients (1, -1)");
   const auto [na, nb] = spin_occupations(orb_);
@@ -182,7 +183,7 @@ ScfOutcome DfPlan::solve_unrestricted(const ScfOptions& o, const std::vector<dou
 
   // Stacked [alpha; beta] arrays: densities, Fock matrices and orbitals.
   DeviceArray<double> X(nn), V(nn), w2(2 * static_cast<std::size_t>(n)), e(n), T1(nn), T2(nn);
-  DeviceArray<double> D(2 * nn), Dn(2 * nn), F(2 * nn), Fe(2 * nn), Jt(nn), Ka(nn), W(2 * nn), Ew(nn);
+  DeviceArray<double> D(2 * nn), Dn(2 * nn), F(2 * nn), Fe(2 * nn), Jt(nn), Ka(nn), W(2 * nn);
   DeviceArray<rocblas_int> info(1);
   DeviceArray<double> scalars(8);
   auto eigh = [&](const double* A, double* vectors, double* values) {
@@ -397,36 +398,53 @@ ScfOutcome DfPlan::solve_unrestricted(const ScfOptions& o, const std::vector<dou
   sync();
   out.energy = 0.5 * (hs[0] + hs[1] + hs[2] + hs[3]) + enuc_;
   out.converged = std::isfinite(out.energy);
-  if (forces && out.converged) {
-    // -dE/dR (analytic_uhf_forces): one-electron terms of the total density
-    // and the total energy-weighted density E = sum_s C_s diag(e_s) C_s^T,
-    // the DF two-electron derivative, and the nuclear repulsion.
-    const std::size_t ng = orb_.atoms.size() * 3;
-    DeviceArray<double> g(ng);
-    VQC_HIP(hipMemsetAsync(g.data(), 0, g.bytes(), stream_));
-    for (int s = 0; s < 2; ++s) {
-      hipLaunchKernelGGL(weighted_columns_kernel, dim3(blocks_for(nn)), dim3(256), 0, stream_, W.data() + s * nn,
-                         w2.data() + static_cast<std::size_t>(s) * n, n, occ[s], T1.data());
-      VQC_LAUNCH("weighted_columns_kernel");
-      const double b = s == 0 ? 0.0 : 1.0;
-      VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_transpose, n, n, std::max(occ[s], 1), &one,
-                             T1.data(), n, W.data() + s * nn, n, &b, Ew.data(), n));
-    }
-    assemble_fock(D.data(), D.data() + nn, D.data() + nn, 1.0, 0.0, nn, T2.data(), stream_);
-    compute_one_electron_gradient(orb_, ob_, boys_.data(), T2.data(), Ew.data(), g.data(), stream_);
-    const double* orbitals[2] = {W.data(), W.data() + nn};
-    unrestricted_two_electron_gradient(D.data(), D.data() + nn, orbitals, occ, g.data());
-    std::vector<double> hg(ng);
-    g.download(hg.data(), ng, stream_);
-    sync();
-    const auto gn = nuclear_repulsion_gradient();
-    out.forces.resize(ng);
-    for (std::size_t k = 0; k < ng; ++k) {
-      out.forces[k] = -(hg[k] + gn[k]);
-      if (!std::isfinite(out.forces[k])) throw NumericalFailure("non-finite analytic force");
-    }
+  if (out.converged) {
+    uhf_orbitals_ = std::move(W);
+    uhf_energies_ = std::move(w2);
+    uhf_density_ = std::move(D);
+    uhf_occ_[0] = na;
+    uhf_occ_[1] = nb;
+    converged_unrestricted_ = true;
+    converged_state_ = true;
   }
+  if (forces && out.converged) out.forces = converged_forces();
   return out;
+}
+
+std::vector<double> DfPlan::unrestricted_forces() {
+  // -dE/dR (analytic_uhf_forces): one-electron terms of the total density and
+  // the total energy-weighted density E = sum_s C_s diag(e_s) C_s^T, the DF
+  // two-electron derivative, and the nuclear repulsion.
+  const int n = n_;
+  const std::size_t nn = static_cast<std::size_t>(n) * n;
+  const double one = 1.0;
+  const double* W = uhf_orbitals_.data();
+  const double* D = uhf_density_.data();
+  const std::size_t ng = orb_.atoms.size() * 3;
+  DeviceArray<double> g(ng), T(nn), E(nn), Dt(nn);
+  VQC_HIP(hipMemsetAsync(g.data(), 0, g.bytes(), stream_));
+  for (int s = 0; s < 2; ++s) {
+    hipLaunchKernelGGL(weighted_columns_kernel, dim3(blocks_for(nn)), dim3(256), 0, stream_, W + s * nn,
+                       uhf_energies_.data() + static_cast<std::size_t>(s) * n, n, uhf_occ_[s], T.data());
+    VQC_LAUNCH("weighted_columns_kernel");
+    const double b = s == 0 ? 0.0 : 1.0;
+    VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_transpose, n, n, std::max(uhf_occ_[s], 1),
+                           &one, T.data(), n, W + s * nn, n, &b, E.data(), n));
+  }
+  assemble_fock(D, D + nn, D + nn, 1.0, 0.0, nn, Dt.data(), stream_);
+  compute_one_electron_gradient(orb_, ob_, boys_.data(), Dt.data(), E.data(), g.data(), stream_);
+  const double* orbitals[2] = {W, W + nn};
+  unrestricted_two_electron_gradient(D, D + nn, orbitals, uhf_occ_, g.data());
+  std::vector<double> hg(ng);
+  g.download(hg.data(), ng, stream_);
+  sync();
+  const auto gn = nuclear_repulsion_gradient();
+  std::vector<double> forces(ng);
+  for (std::size_t k = 0; k < ng; ++k) {
+    forces[k] = -(hg[k] + gn[k]);
+    if (!std::isfinite(forces[k])) throw NumericalFailure("non-finite analytic force");
+  }
+  return forces;
 }
 
 }  // namespace vqc
