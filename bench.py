@@ -20,7 +20,10 @@ Also reported:
     precision (BF16X3 iterations, then FP64 refinement), seconds and energy gap;
   * ragged_batch: 512 (H2O)n clusters, n ~ U{1..16}, molecules/s mixed and FP64;
   * scf_water4: (H2O)4 cc-pVDZ SCF seconds per molecule in FP64 and in mixed
-    precision with the energy gap.
+    precision with the energy gap;
+  * uhf: 4 OH radicals (doublets, cc-pVDZ / cc-pVDZ-JKFIT, 19 AO), UHF
+    energy + forces one plan at a time, systems/s, with the UHF oracle's
+    systems/s on the host beside it.
 
 Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu-baseline]
 """
@@ -335,6 +338,72 @@ def _ragged_cpu_baseline(sample=4, max_waters=6):
     }
 
 
+def _oh_radicals(count=8):
+    """OH doublets with O-H from 1.76 to 1.90 bohr and a small off-axis tilt."""
+    from vibeqc_amd.basis import Atom
+
+    return [(Atom(8, (0.0, 0.0, 0.0)), Atom(1, (0.0, 0.03 * k, 1.76 + 0.02 * k))) for k in range(count)]
+
+
+def _uhf_single(repeats=3):
+    """UHF energy + forces of the OH radicals one plan at a time (cc-pVDZ /
+    cc-pVDZ-JKFIT, 19 AO): the first pass includes plan setup on a cold
+    process, the median of the later passes is reported as systems/s."""
+    from vibeqc_amd import Calculator
+
+    systems = _oh_radicals(4)
+    calc = Calculator(method="uhf", energy_tolerance=1e-12, density_tolerance=1e-10)
+    passes, energies, iterations = [], [], []
+    for rep in range(repeats + 1):
+        t0 = time.perf_counter()
+        for atoms in systems:
+            r = calc.singlepoint(atoms, multiplicity=2)
+            if rep == 0:
+                energies.append(r.energy)
+                iterations.append(r.iterations)
+        passes.append(time.perf_counter() - t0)
+    med = float(np.median(passes[1:]))
+    out = {
+        "workload": "4 OH radicals, UHF doublets, cc-pVDZ (19 AO) / cc-pVDZ-JKFIT, energy + forces, one plan each",
+        "systems": len(systems),
+        "first_pass_s": passes[0],
+        "median_pass_s": med,
+        "systems_per_s": len(systems) / med,
+        "energies": energies,
+        "mean_iterations": float(np.mean(iterations)),
+    }
+    _log(f"UHF single plans: {out}")
+    return out
+
+
+def _uhf_cpu_baseline(sample=4):
+    """The UHF oracle (C integrals, numpy SCF and gradient) on the same
+    radicals, energy + forces."""
+    from oracle import oracle as O
+    from vibeqc_amd import shells_for
+
+    def oshells(shells):
+        return [O.Shell(s.atom_index, s.l, tuple(p.exponent for p in s.primitives), tuple(p.coefficient for p in s.primitives)) for s in shells]
+
+    systems = _oh_radicals()[:sample]
+    t0 = time.perf_counter()
+    for atoms in systems:
+        z = [a.atomic_number for a in atoms]
+        xyz = np.array([a.position for a in atoms])
+        orb, aux = oshells(shells_for("cc-pvdz", atoms)), oshells(shells_for("cc-pvdz-jkfit", atoms))
+        problem = O.build_problem(z, xyz, orb, True, aux, True)
+        r = O.df_uhf(problem, 5, 4, energy_tolerance=1e-12, density_tolerance=1e-10)
+        O.df_uhf_gradient(z, xyz, orb, True, aux, True, problem, r, (5, 4))
+    dt = time.perf_counter() - t0
+    return {
+        "value": sample / dt,
+        "unit": "systems/s",
+        "cores": os.cpu_count() or 1,
+        "kind": "port",
+        "sample": f"UHF oracle energy + forces on the {sample} radicals",
+    }
+
+
 def _scf_stats(plan, result, seconds):
     st = plan.last_solve_stats()
     return {"energy": result.energy, "iterations": result.iterations, "fock_builds": result.fock_builds,
@@ -395,7 +464,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 PMC byte-count runs")
-    ap.add_argument("--no-batch", action="store_true", help="skip the 512-molecule ragged batch")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-molecule ragged batch and the UHF runs")
     ap.add_argument("--no-scf64", action="store_true", help="skip the (H2O)64 FP64 and mixed SCF runs")
     ap.add_argument(
         "--only",
@@ -456,6 +525,12 @@ def main():
         if not args.no_cpu_baseline:
             ragged["cpu_baseline"] = _ragged_cpu_baseline()
             _log(f"ragged cpu baseline: {ragged['cpu_baseline']}")
+    uhf = None
+    if not args.no_batch:
+        uhf = _uhf_single()
+        if not args.no_cpu_baseline:
+            uhf["cpu_baseline"] = _uhf_cpu_baseline()
+            _log(f"UHF cpu baseline: {uhf['cpu_baseline']}")
     traffic = traffic64 = None
     if not args.no_traffic:
         traffic = _pmc_traffic("c20h42", "bf16x3")
@@ -505,6 +580,7 @@ def main():
             "scf": w64_scf,
         },
         "ragged_batch": ragged,
+        "uhf": uhf,
         "scf_water4": {
             "fp64": scf64,
             "mixed": scfmx,
