@@ -11,7 +11,10 @@ Reads (as data only; nothing is imported or run from the reference):
     -- the reference's checksummed canonical basis records (copied unchanged);
   * tests/reference_data/validation/hf-plus-uhf.json -- PySCF 2.6.2
     exact-integral UHF of HF+ (doublet, STO-3G Cartesian): energy, densities,
-    one-electron matrices and orbital energies.
+    one-electron matrices and orbital energies;
+  * tests/data/external_basis/{manifest.json,oracles.npz} -- the water STO-3G
+    overlap, kinetic and core-Hamiltonian matrices (Cartesian and spherical,
+    two geometries).
 The libcint scalar known answers of tests/native/test_density_fitting.cpp:468-499
 are copied into libcint_known_answers.json by hand (they are literals in that
 test, not a data file).
@@ -82,6 +85,23 @@ def main(reference: str) -> None:
         "versions": {"pyscf": meta["provenance"]["pyscf"]},
     }
     (HERE / "uhf_hf_plus.json").write_text(json.dumps(record, indent=1) + "\n")
+
+    # External-basis oracle: water STO-3G one-electron matrices (BSE basis),
+    # Cartesian and spherical, at the case geometry and a changed geometry.
+    ext = root / "tests/data/external_basis"
+    manifest = json.loads((ext / "manifest.json").read_text())
+    arrays = np.load(ext / "oracles.npz", allow_pickle=False)
+    record = {}
+    for case in manifest["cases"]:
+        name = case["name"]
+        if not name.startswith("water_"):
+            continue
+        entry = {"representation": case["representation"], "atoms": case["atoms"], "changed_atoms": case["changed_atoms"]}
+        for tag in ("", "changed_"):
+            for key in ("overlap", "kinetic", "hcore"):
+                entry[tag + key] = arrays[f"{name}_{tag}{key}"].tolist()
+        record[name] = entry
+    (HERE / "external_water_sto3g.json").write_text(json.dumps(record, indent=1) + "\n")
 
     identity = root / "benchmarks/results/issue206-practical-auxiliary/identity"
     for name in ("cc-pvdz", "cc-pvdz-jkfit"):
