@@ -188,3 +188,27 @@ def test_uhf_batch_matches_single_plans_and_checkpoints(tmp_path):
         assert b.restart_origin == "persistent_restart"
         assert abs(a.energy - b.energy) < 1e-9
         assert b.iterations <= 3
+
+
+def test_near_degenerate_radical_state_follows_the_seed(oracle):
+    """(H2O)2 ... OH: the OH pi pair is nearly degenerate, so the core guess's
+    frontier rotation acts inside an almost arbitrary eigenbasis and cold
+    starts may reach different UHF stationary points (the oracle's numpy and
+    the device's rocSOLVER eigenvectors differ there).  Seeded with the
+    oracle's converged densities, the device SCF stays at the oracle's state."""
+    from vibeqc_amd import Calculator
+    from vibeqc_amd.basis import Atom
+    from vibeqc_amd.geometry import water_cluster
+
+    atoms = tuple(water_cluster(2, 3.5, seed=11)) + (Atom(8, (7.0, 0.0, 0.0)), Atom(1, (7.0, 0.0, 1.83)))
+    case = molecule_case("cluster", atoms)
+    case.multiplicity = 2
+    p = case.oracle_problem(oracle)
+    na, nb = oracle.spin_occupations(_electrons(case), 2)
+    ref = oracle.df_uhf(p, na, nb, **TIGHT)
+    calc = Calculator(method="uhf", energy_tolerance=1e-12, density_tolerance=1e-10)
+    with calc.prepare(atoms, multiplicity=2) as plan:
+        r = plan.solve(initial_density=ref.density, energy_tolerance=1e-12, density_tolerance=1e-10)
+    print(f"seeded E {r.energy:.12f} oracle {ref.energy:.12f} it={r.iterations}")
+    assert abs(r.energy - ref.energy) < 1e-9
+    assert np.abs(np.asarray(r.density) - ref.density).max() < 1e-6
