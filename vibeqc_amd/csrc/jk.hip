@@ -691,7 +691,7 @@ template <class T>
 __global__ __launch_bounds__(512) void exchange_syrk_kernel(const T* __restrict__ W, std::size_t kd, int n,
                                                            int ntiles, int ncb, int nwork, std::size_t kbegin,
                                                            std::size_t kend, std::size_t kchunk, double sign,
-                                                           double* __restrict__ partial);
+                                                           T* __restrict__ partial);
 
 // BF16X3 SYRK over the pre-split hi/lo planes written by occupied_u.
 // Block: 8 waves (4 x 2), tile 256 (mu) x 128 (lambda); each wave owns 64 x 64
@@ -728,7 +728,7 @@ __global__ __launch_bounds__(512) void exchange_syrk_bf16x3_kernel(const __bf16*
                                                                   int n, int ntiles, int ncb, int nwork,
                                                                   std::size_t kbegin, std::size_t kend,
                                                                   std::size_t kchunk, double sign,
-                                                                  double* __restrict__ partial) {
+                                                                  float* __restrict__ partial) {
   constexpr int T = kSyrkBfT, BK = 32, PAD = 8, NT = 512, XF = 8, YF = 4;
   __shared__ __bf16 Ah[T][BK + PAD], Al[T][BK + PAD];
   __shared__ __bf16 Bh[T][BK + PAD], Bl[T][BK + PAD];
@@ -815,7 +815,9 @@ __global__ __launch_bounds__(512) void exchange_syrk_bf16x3_kernel(const __bf16*
       __syncthreads();
     }
   }
-  double* out = partial + static_cast<std::size_t>(split) * n * n;
+  // FP32 partials hold the FP32 accumulators exactly (sign is +-1).
+  float* out = partial + static_cast<std::size_t>(split) * n * n;
+  const float fsign = static_cast<float>(sign);
 #pragma unroll
   for (int i = 0; i < XF; ++i)
 #pragma unroll
@@ -824,8 +826,7 @@ __global__ __launch_bounds__(512) void exchange_syrk_bf16x3_kernel(const __bf16*
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int mu = m0 + wr * 16 * XF + i * 16 + 4 * (lane >> 4) + v;
-        if (mu < n && lam < n && lam <= mu)
-          out[static_cast<std::size_t>(mu) * n + lam] = sign * static_cast<double>(acc[i][j][v]);
+        if (mu < n && lam < n && lam <= mu) out[static_cast<std::size_t>(mu) * n + lam] = fsign * acc[i][j][v];
       }
     }
 }
@@ -834,7 +835,7 @@ template <class T>
 __global__ __launch_bounds__(512) void exchange_syrk_kernel(const T* __restrict__ W, std::size_t kd, int n,
                                                            int ntiles, int ncb, int nwork, std::size_t kbegin,
                                                            std::size_t kend, std::size_t kchunk, double sign,
-                                                           double* __restrict__ partial) {
+                                                           T* __restrict__ partial) {
   using Acc = typename AccOf<T>::type;
   constexpr int TM = kSyrkTM, TN = kSyrkTN, BK = 16, PAD = 1, NT = 512, V = 16 / sizeof(T);
   __shared__ T As[2][TM][BK + PAD];
@@ -911,7 +912,9 @@ __global__ __launch_bounds__(512) void exchange_syrk_kernel(const T* __restrict_
     __syncthreads();
     cur ^= 1;
   }
-  double* out = partial + static_cast<std::size_t>(split) * n * n;
+  // Partials in the accumulator's own type hold it exactly (sign is +-1).
+  T* out = partial + static_cast<std::size_t>(split) * n * n;
+  const T tsign = static_cast<T>(sign);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -920,20 +923,20 @@ __global__ __launch_bounds__(512) void exchange_syrk_kernel(const T* __restrict_
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int mu = m0 + wr * 64 + i * 16 + acc_row<T>(lane, v);
-        if (mu < n && lam < n && lam <= mu)
-          out[static_cast<std::size_t>(mu) * n + lam] = sign * static_cast<double>(acc[i][j][v]);
+        if (mu < n && lam < n && lam <= mu) out[static_cast<std::size_t>(mu) * n + lam] = tsign * acc[i][j][v];
       }
     }
 }
 
-__global__ void reduce_exchange_kernel(const double* __restrict__ partial, int slices, int n,
-                                       double* __restrict__ K) {
+// K = sum of the split partials (FP64 sum in slice order), mirrored.
+template <class P>
+__global__ void reduce_exchange_kernel(const P* __restrict__ partial, int slices, int n, double* __restrict__ K) {
   const std::size_t e = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= static_cast<std::size_t>(n) * n) return;
   const int mu = static_cast<int>(e / n), lam = static_cast<int>(e % n);
   if (lam > mu) return;
   double s = 0.0;
-  for (int k = 0; k < slices; ++k) s += partial[static_cast<std::size_t>(k) * n * n + e];
+  for (int k = 0; k < slices; ++k) s += static_cast<double>(partial[static_cast<std::size_t>(k) * n * n + e]);
   K[static_cast<std::size_t>(mu) * n + lam] = s;
   K[static_cast<std::size_t>(lam) * n + mu] = s;
 }
@@ -1144,7 +1147,10 @@ void exchange_syrk(Precision prec, const void* U, int n, int r, int r_pos, int l
   auto run = [&](std::size_t kb, std::size_t ke, double sign) {
     if (ke <= kb) return;
     const std::size_t len = ke - kb;
-    double* out = partial + static_cast<std::size_t>(slices) * n * n;
+    // FP64 partials for the FP64 build, FP32 (the accumulators' type) otherwise.
+    const std::size_t at = static_cast<std::size_t>(slices) * n * n;
+    double* out64 = partial + at;
+    float* out32 = reinterpret_cast<float*>(partial) + at;
     int ncb = 0;
     const bool bf = prec == Precision::BF16X3;
     const int tiles = bf ? syrk_tiles(n, kSyrkBfT, kSyrkBfT, ncb) : syrk_tiles(n, kSyrkTM, kSyrkTN, ncb);
@@ -1161,13 +1167,13 @@ void exchange_syrk(Precision prec, const void* U, int n, int r, int r_pos, int l
       const __bf16* wh = static_cast<const __bf16*>(U);
       const __bf16* wl = wh + static_cast<std::size_t>(n) * kd;
       hipLaunchKernelGGL(exchange_syrk_bf16x3_kernel, dim3(grid), dim3(512), 0, s, wh, wl, kd, n, tiles, ncb, nwork,
-                         kb, ke, chunk, sign, out);
+                         kb, ke, chunk, sign, out32);
     } else if (prec == Precision::FP64) {
       hipLaunchKernelGGL(exchange_syrk_kernel<double>, dim3(grid), dim3(512), 0, s, static_cast<const double*>(U), kd,
-                         n, tiles, ncb, nwork, kb, ke, chunk, sign, out);
+                         n, tiles, ncb, nwork, kb, ke, chunk, sign, out64);
     } else {
       hipLaunchKernelGGL(exchange_syrk_kernel<float>, dim3(grid), dim3(512), 0, s, static_cast<const float*>(U), kd,
-                         n, tiles, ncb, nwork, kb, ke, chunk, sign, out);
+                         n, tiles, ncb, nwork, kb, ke, chunk, sign, out32);
     }
     VQC_LAUNCH("exchange_syrk_kernel");
     slices += splits;
@@ -1179,7 +1185,11 @@ void exchange_syrk(Precision prec, const void* U, int n, int r, int r_pos, int l
     return;
   }
   const std::size_t e = static_cast<std::size_t>(n) * n;
-  hipLaunchKernelGGL(reduce_exchange_kernel, dim3((e + 255) / 256), dim3(256), 0, s, partial, slices, n, K);
+  if (prec == Precision::FP64)
+    hipLaunchKernelGGL(reduce_exchange_kernel<double>, dim3((e + 255) / 256), dim3(256), 0, s, partial, slices, n, K);
+  else
+    hipLaunchKernelGGL(reduce_exchange_kernel<float>, dim3((e + 255) / 256), dim3(256), 0, s,
+                       reinterpret_cast<const float*>(partial), slices, n, K);
   VQC_LAUNCH("reduce_exchange_kernel");
 }
 
