@@ -482,8 +482,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(
-            "bench.py measures one MI355X: this engine has no multi-GPU path in this build "
-            "(sharding over the auxiliary index is described in DESIGN.md). Run with --gpus 1.",
+            "bench.py measures one MI355X: the engine runs on one GPU (a multi-GPU run would "
+            "shard the auxiliary index and all-reduce J and K). Run with --gpus 1.",
             file=sys.stderr,
         )
         sys.exit(2)

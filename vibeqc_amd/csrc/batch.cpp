@@ -21,7 +21,7 @@ namespace vqc {
 namespace {
 
 // Keep freed stream-ordered memory in the device pool for the next plan
-// rather than returning it to the driver at every synchronisation.
+// rather than releasing it back to the system at every synchronisation.
 void retain_pool(int device) {
   hipMemPool_t pool = nullptr;
   if (hipDeviceGetDefaultMemPool(&pool, device) != hipSuccess || !pool) return;
