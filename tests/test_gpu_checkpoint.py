@@ -81,3 +81,32 @@ def test_incompatible_basis_and_clear(tmp_path):
         batch.clear_warm_starts()
         result = batch.execute(properties=("energy",), strict=True)
         assert all(it.restart_origin == "cold" for it in result.items)
+
+
+def test_empty_and_refused_restores(tmp_path):
+    """A batch that has not executed saves slots without seeds (status -1);
+    restoring into a batch without warm starts is refused before any change."""
+    from vibeqc_amd import Calculator
+    from vibeqc_amd.checkpoint import CheckpointError, inspect_checkpoint
+
+    systems = _systems()[:2]
+    empty = tmp_path / "empty.vqcp"
+    full = tmp_path / "full.vqcp"
+    calc = Calculator()
+    with calc.prepare_batch(systems) as batch:
+        report = batch.save_checkpoint(empty)
+        assert report["saved_indices"] == []
+        batch.execute(properties=("energy",), strict=True)
+        batch.save_checkpoint(full)
+    doc = inspect_checkpoint(empty)
+    assert all(it["seed"] is None and it["status"] == -1 for it in doc["items"]) and doc["blobs"] == []
+    with calc.prepare_batch(systems) as batch:
+        rep = batch.load_checkpoint(empty, strict=False)
+        assert all(r["compatibility"] == "incompatible" for r in rep["items"])
+        with pytest.raises(CheckpointError, match="missing reusable state"):
+            batch.load_checkpoint(empty)
+    with calc.prepare_batch(systems, warm_start=False) as batch:
+        with pytest.raises(CheckpointError, match="native"):
+            batch.load_checkpoint(full)
+        result = batch.execute(properties=("energy",), strict=True)
+        assert all(it.restart_origin == "cold" for it in result.items)
