@@ -350,15 +350,29 @@ struct OneElectronArgs {
   double* T;
   double* V;
   int n;
+  int sub;  // lanes per shell pair (1, 2, 4 or 8): they split the nuclei of V
 };
+
+// Lanes per shell pair for the one-electron kernel: enough that each lane
+// takes about four nuclei of the V sum, at most eight.
+inline int one_electron_lanes(int natom) {
+  int sub = 1;
+  while (sub < 8 && sub * 4 < natom) sub *= 2;
+  return sub;
+}
 
 template <int LA, int LB>
 __global__ __launch_bounds__(64) void one_electron_kernel(const OneElectronArgs a) {
   constexpr int NA = nc(LA), NB = nc(LB), L = LA + LB;
   constexpr int JB = LB + 2, TD = LA + JB + 1;
+  // a.sub consecutive lanes (one aligned group of a wave) share a shell pair:
+  // each takes every a.sub-th nucleus of V, and the group's V is summed by a
+  // fixed butterfly.  Lanes past the last pair compute pair 0 and write
+  // nothing, so every lane reaches the shuffles.
   const int id = blockIdx.x * 64 + threadIdx.x;
-  if (id >= a.ntask) return;
-  const int2 task = a.tasks[id];
+  const int pair = id / a.sub, part = id % a.sub;
+  const bool active = pair < a.ntask;
+  const int2 task = a.tasks[active ? pair : 0];
   const DevShell A = a.shells[task.x];
   const DevShell B = a.shells[task.y];
   double s[NA][NB], k[NA][NB], v[NA][NB];
@@ -429,7 +443,7 @@ __global__ __launch_bounds__(64) void one_electron_kernel(const OneElectronArgs 
           s[i][j] += kab * sd[0] * sd[1] * sd[2];
           k[i][j] += kab * (kd[0] * sd[1] * sd[2] + sd[0] * kd[1] * sd[2] + sd[0] * sd[1] * kd[2]);
         }
-      for (int c = 0; c < a.natom; ++c) {
+      for (int c = part; c < a.natom; c += a.sub) {
         const double4 at = a.atoms[c];
         const double X = P[0] - at.x, Y = P[1] - at.y, Z = P[2] - at.z;
         double F[L + 1];
@@ -456,6 +470,12 @@ __global__ __launch_bounds__(64) void one_electron_kernel(const OneElectronArgs 
       }
     }
   }
+  for (int off = 1; off < a.sub; off <<= 1)
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) v[i][j] += __shfl_xor(v[i][j], off);
+  if (!active || part != 0) return;
   // Public AOs: out = TA x TB^T, written to both triangles.
   for (int ma = 0; ma < a.nfa; ++ma)
     for (int mb = 0; mb < a.nfb; ++mb) {
@@ -507,7 +527,11 @@ void dispatch_lc(int lc, bool cart, const ThreeCenterArgs& args, int ntask, hipS
 template <int LA, int LB>
 void launch_1e(const OneElectronArgs& args, hipStream_t s) {
   if (args.ntask == 0) return;
-  hipLaunchKernelGGL((one_electron_kernel<LA, LB>), dim3((args.ntask + 63) / 64), dim3(64), 0, s, args);
+  if (args.sub < 1 || args.sub > 8 || (args.sub & (args.sub - 1)) != 0)
+    throw std::invalid_argument("one-electron lanes per pair must be 1, 2, 4 or 8");
+  const std::size_t lanes = static_cast<std::size_t>(args.ntask) * args.sub;
+  hipLaunchKernelGGL((one_electron_kernel<LA, LB>), dim3(static_cast<unsigned>((lanes + 63) / 64)), dim3(64), 0, s,
+                     args);
   VQC_LAUNCH("one_electron_kernel");
 }
 
