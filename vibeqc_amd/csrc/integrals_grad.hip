@@ -263,7 +263,8 @@ void compute_one_electron_gradient(const System& sys, const DeviceBasis& basis, 
     if (classes[key].empty()) continue;
     const int la = key / 10, lb = key % 10;
     const int ntask = static_cast<int>(classes[key].size());
-    const int nblock = (ntask + 63) / 64;
+    const int sub = one_electron_lanes(natom);
+    const int nblock = one_electron_grad_blocks(ntask, sub);
     std::vector<int2> ta(ntask);
     for (int t = 0; t < ntask; ++t)
       ta[t] = make_int2(static_cast<int>(sys.shells[classes[key][t].x].atom),
@@ -292,6 +293,7 @@ void compute_one_electron_gradient(const System& sys, const DeviceBasis& basis, 
     args.n = static_cast<int>(basis.nfunctions);
     args.out_ab = p.ab.data();
     args.out_c = p.c.data();
+    args.sub = sub;
     dispatch_1e_grad(la, lb, args, s);
     hipLaunchKernelGGL(reduce_grad_kernel, dim3(natom), dim3(kReduceThreads), 0, s, p.ab.data(), ntask, 1,
                        p.task_atoms.data(), p.c.data(), nblock, natom, nuclei.data(), grad);

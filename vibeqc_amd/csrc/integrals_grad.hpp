@@ -288,6 +288,7 @@ struct OneElectronGradArgs {
   int n;
   double* out_ab;  // [ntask][6]
   double* out_c;   // [gridDim.x][natom][3]: Hellmann-Feynman term of each nucleus
+  int sub;         // lanes per shell pair (1, 2, 4 or 8): they split the nuclei
 };
 
 // Two-index Hermite tables E[i][j][t] along one axis, i <= I, j <= J.
@@ -329,13 +330,20 @@ __device__ __forceinline__ void hermite_ij(double pa, double pb, double h, doubl
 // the overlap and kinetic parts through d/dB = -d/dA, the nuclear attraction
 // through d/dA and d/dB of each nucleus' term and its Hellmann-Feynman
 // derivative -(d/dA + d/dB) (s_integrals.cpp:1865-2060; hf_gradient.cpp:8-18).
+//
+// a.sub consecutive lanes (an aligned group) share a shell pair: in round r
+// lane `part` of the group takes nucleus r * sub + part, so every lane works
+// in every round.  The Hellmann-Feynman term of a nucleus is summed over the
+// lanes of the wave with the same part, the pair's dA/dB over its group,
+// both by fixed butterflies; the overlap/kinetic terms count once (part 0).
 template <int LA, int LB>
 __global__ __launch_bounds__(64) void one_electron_grad_kernel(const OneElectronGradArgs a) {
   constexpr int NA = nc(LA), NB = nc(LB), L = LA + LB + 1;
   constexpr int I = LA + 1, J = LB + 2;
   const int id = blockIdx.x * 64 + threadIdx.x;
-  const bool active = id < a.ntask;
-  const int2 task = a.tasks[active ? id : 0];
+  const int pair = id / a.sub, part = id % a.sub;
+  const bool active = pair < a.ntask;
+  const int2 task = a.tasks[active ? pair : 0];
   const DevShell A = a.shells[task.x];
   const DevShell B = a.shells[task.y];
   double wd[NA][NB], we[NA][NB];
@@ -425,11 +433,16 @@ __global__ __launch_bounds__(64) void one_electron_grad_kernel(const OneElectron
     }
   }
 #pragma unroll
-  for (int d = 0; d < 3; ++d) gB[d] = -gA[d];
-  // Nuclear attraction, one nucleus at a time so the Hellmann-Feynman term can
-  // be reduced across the wave (every lane runs the same nucleus loop).
-  for (int c = 0; c < a.natom; ++c) {
-    const double4 at = a.atoms[c];
+  for (int d = 0; d < 3; ++d) {
+    if (part != 0) gA[d] = 0.0;
+    gB[d] = -gA[d];
+  }
+  // Nuclear attraction, one nucleus per lane and round so the Hellmann-Feynman
+  // term can be reduced across the wave (every lane runs the same rounds).
+  for (int c0 = 0; c0 < a.natom; c0 += a.sub) {
+    const int c = c0 + part;
+    const bool has = c < a.natom;
+    const double4 at = a.atoms[has ? c : 0];
     double dA[3] = {0.0, 0.0, 0.0}, dB[3] = {0.0, 0.0, 0.0};
     for (int ia = 0; ia < A.nprim; ++ia) {
       const double ea = a.exps[A.prim_off + ia], ca = a.coefs[A.prim_off + ia];
@@ -454,7 +467,7 @@ __global__ __launch_bounds__(64) void one_electron_grad_kernel(const OneElectron
           for (int j = 0; j < NB; ++j) {
             const int ai[3] = {cart_lx(LA, i), cart_ly(LA, i), cart_lz(LA, i)};
             const int bj[3] = {cart_lx(LB, j), cart_ly(LB, j), cart_lz(LB, j)};
-            const double w = pref * wd[i][j];
+            const double w = has ? pref * wd[i][j] : 0.0;
 #pragma unroll
             for (int which = 0; which < 2; ++which)
 #pragma unroll
@@ -484,12 +497,19 @@ __global__ __launch_bounds__(64) void one_electron_grad_kernel(const OneElectron
     for (int d = 0; d < 3; ++d) {
       gA[d] += dA[d];
       gB[d] += dB[d];
-      const double hf = wave_sum(-(dA[d] + dB[d]));
-      if (threadIdx.x == 0) a.out_c[(static_cast<std::size_t>(blockIdx.x) * a.natom + c) * 3 + d] = hf;
+      double hf = -(dA[d] + dB[d]);
+      for (int off = 32; off >= a.sub; off >>= 1) hf += __shfl_xor(hf, off);
+      if (threadIdx.x < a.sub && has) a.out_c[(static_cast<std::size_t>(blockIdx.x) * a.natom + c) * 3 + d] = hf;
     }
   }
-  if (active) {
-    double* o = a.out_ab + static_cast<std::size_t>(id) * 6;
+  for (int off = 1; off < a.sub; off <<= 1)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      gA[d] += __shfl_xor(gA[d], off);
+      gB[d] += __shfl_xor(gB[d], off);
+    }
+  if (active && part == 0) {
+    double* o = a.out_ab + static_cast<std::size_t>(pair) * 6;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       o[d] = gA[d];
@@ -523,10 +543,18 @@ void dispatch_lc_grad(int lc, bool cart, const ThreeCenterGradArgs& args, int nc
   }
 }
 
+// Blocks of the one-electron gradient launch for `ntask` pairs at `sub` lanes each.
+inline int one_electron_grad_blocks(int ntask, int sub) {
+  return static_cast<int>((static_cast<std::size_t>(ntask) * sub + 63) / 64);
+}
+
 template <int LA, int LB>
 void launch_1e_grad(const OneElectronGradArgs& args, hipStream_t s) {
   if (args.ntask == 0) return;
-  hipLaunchKernelGGL((one_electron_grad_kernel<LA, LB>), dim3((args.ntask + 63) / 64), dim3(64), 0, s, args);
+  if (args.sub < 1 || args.sub > 8 || (args.sub & (args.sub - 1)) != 0)
+    throw std::invalid_argument("one-electron lanes per pair must be 1, 2, 4 or 8");
+  hipLaunchKernelGGL((one_electron_grad_kernel<LA, LB>), dim3(one_electron_grad_blocks(args.ntask, args.sub)),
+                     dim3(64), 0, s, args);
   VQC_LAUNCH("one_electron_grad_kernel");
 }
 
