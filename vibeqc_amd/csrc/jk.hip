@@ -563,7 +563,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void o
 #pragma unroll
     for (int y = 0; y < YF; ++y) acc[x][y] = f4{};
   double racc[XF] = {0.0, 0.0};
-  fl2 rx[8];
+  fl2 rx[8] = {};
   bf8 rch[YP], rcl[YP];
   const float* Bq = B + q_off;
   auto gload = [&](int nu0) {
@@ -593,8 +593,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void o
     }
   };
   __syncthreads();  // rowoff and drow
+  // Waves past ldq (only in the last Q tile) skip the gathers and MFMAs; they
+  // still stage their share of C^T and meet every barrier.
   cload(0);
-  gload(0);
+  if (q_ok) gload(0);
   cstore(0);
   __syncthreads();
   int cur = 0;
@@ -631,20 +633,22 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void o
     const bool more = nu0 + BK < n;
     if (more) {
       cload(nu0 + BK);
-      gload(nu0 + BK);
+      if (q_ok) gload(nu0 + BK);
     }
     const int kq = 8 * kg;
+    if (q_ok) {
 #pragma unroll
-    for (int y = 0; y < YF; ++y) {
-      const int row = y * 16 + (lane & 15);
-      const bf8 bh = *reinterpret_cast<const bf8*>(&Yh[cur][row][kq]);
-      const bf8 bl = *reinterpret_cast<const bf8*>(&Yl[cur][row][kq]);
+      for (int y = 0; y < YF; ++y) {
+        const int row = y * 16 + (lane & 15);
+        const bf8 bh = *reinterpret_cast<const bf8*>(&Yh[cur][row][kq]);
+        const bf8 bl = *reinterpret_cast<const bf8*>(&Yl[cur][row][kq]);
 #pragma unroll
-      for (int x = 0; x < XF; ++x) acc[x][y] = mfma(ah[x], bh, acc[x][y]);
+        for (int x = 0; x < XF; ++x) acc[x][y] = mfma(ah[x], bh, acc[x][y]);
 #pragma unroll
-      for (int x = 0; x < XF; ++x) acc[x][y] = mfma(ah[x], bl, acc[x][y]);
+        for (int x = 0; x < XF; ++x) acc[x][y] = mfma(ah[x], bl, acc[x][y]);
 #pragma unroll
-      for (int x = 0; x < XF; ++x) acc[x][y] = mfma(al[x], bh, acc[x][y]);
+        for (int x = 0; x < XF; ++x) acc[x][y] = mfma(al[x], bh, acc[x][y]);
+      }
     }
     // The other buffer was last read before the previous barrier.
     if (more) cstore(cur ^ 1);
