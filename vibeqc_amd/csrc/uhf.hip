@@ -209,6 +209,58 @@ ScfOutcome DfPlan::solve_unrestricted(const ScfOptions& o, const std::vector<dou
     eigh(T2.data(), V.data(), w2.data() + static_cast<std::size_t>(s) * n);
     gemm(rocblas_operation_none, rocblas_operation_none, n, X.data(), n, V.data(), n, W.data() + s * nn);
   };
+  // Occupied block of F_s during the iterations: SP2 purification of
+  // F~ = X F_s X and a pivoted Cholesky factor G of the projector (the first
+  // n_s columns of W_s are X G), as in the restricted solve; a spin whose
+  // purification does not reach an integer trace, or whose factor is rank
+  // deficient, is diagonalised instead.  The core guess (its frontier
+  // rotation needs the virtual orbital) and the finalisation diagonalise.
+  ScfOutcome out;
+  const int omax = std::max(na, nb);
+  const bool sp2_spin[2] = {na > 0 && na < n && pivoted_cholesky_fits(n, na),
+                            nb > 0 && nb < n && pivoted_cholesky_fits(n, nb)};
+  DeviceArray<double> G(static_cast<std::size_t>(n) * std::max(omax, 1)), sp2_state(kSp2StateSize),
+      sp2_partial(sp2_partial_size(n));
+  DeviceArray<int> sp2_status(1);
+  int sp2_hint[2] = {0, 0};
+  auto purify = [&](int s) {  // T2 -> projector in V; T1 is scratch
+    const double target = static_cast<double>(occ[s]);
+    sp2_begin(T2.data(), n, sp2_partial.data(), sp2_state.data(), V.data(), stream_);
+    int steps = 0, batch = sp2_hint[s] > 0 ? sp2_hint[s] : 32;
+    double hstate[8];
+    for (;;) {
+      for (int j = 0; j < batch && steps < kSp2MaxSteps; ++j, ++steps) {
+        gemm(rocblas_operation_none, rocblas_operation_none, n, V.data(), n, V.data(), n, T1.data());
+        sp2_step(V.data(), T1.data(), n, target, sp2_state.data(), steps, stream_);
+      }
+      sp2_state.download(hstate, 8, stream_);
+      const Sp2Summary sum = sp2_summary(hstate, steps);
+      if (sum.converged && std::abs(sum.trace - target) < 1e-6) {
+        sp2_hint[s] = sum.applied;
+        return true;
+      }
+      if (sum.converged || sum.degenerate || steps >= kSp2MaxSteps) return false;
+      batch = 4;
+    }
+  };
+  auto occupied_of = [&](const double* Fs, int s) {
+    if (sp2_spin[s]) {
+      gemm(rocblas_operation_none, rocblas_operation_none, n, Fs, n, X.data(), n, T1.data());
+      gemm(rocblas_operation_none, rocblas_operation_none, n, X.data(), n, T1.data(), n, T2.data());
+      if (purify(s)) {
+        pivoted_cholesky(V.data(), n, occ[s], G.data(), sp2_status.data(), stream_);
+        int status = 1;
+        sp2_status.download(&status, 1, stream_);
+        if (status == 0) {
+          VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, n, occ[s], n, &one, X.data(),
+                                 n, G.data(), n, &zero, W.data() + s * nn, n));
+          ++out.purified_iterations;
+          return;
+        }
+      }
+    }
+    orbitals_of(Fs, s);
+  };
   // X = S^-1/2 (src/scf/reference/linalg.cpp:123-142).
   eigh(S_.data(), V.data(), w2.data());
   {
@@ -220,7 +272,6 @@ ScfOutcome DfPlan::solve_unrestricted(const ScfOptions& o, const std::vector<dou
   scale_columns_inv_sqrt(V.data(), w2.data(), n, T1.data(), stream_);
   gemm(rocblas_operation_none, rocblas_operation_transpose, n, T1.data(), n, V.data(), n, X.data());
 
-  ScfOutcome out;
   // rank[s] < 0: the factor of D_s comes from an eigendecomposition (seeds).
   int rank[2] = {na, nb};
   if (initial_density) {
@@ -333,7 +384,7 @@ ScfOutcome DfPlan::solve_unrestricted(const ScfOptions& o, const std::vector<dou
       }
     }
     for (int s = 0; s < 2; ++s) {
-      orbitals_of(Fuse + s * nn, s);
+      occupied_of(Fuse + s * nn, s);
       density_of(W.data() + s * nn, occ[s], spin_block(Dn, s));
     }
     from_orbitals = true;
