@@ -23,7 +23,8 @@ Also reported:
     precision with the energy gap;
   * uhf: 4 OH radicals (doublets, cc-pVDZ / cc-pVDZ-JKFIT, 19 AO), UHF
     energy + forces one plan at a time, systems/s, with the UHF oracle's
-    systems/s on the host beside it.
+    systems/s on the host beside it; and the UHF SCF seconds of (H2O)8 ... OH
+    (211 AO) on a prepared plan.
 
 Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu-baseline]
 """
@@ -363,6 +364,17 @@ def _uhf_single(repeats=3):
                 iterations.append(r.iterations)
         passes.append(time.perf_counter() - t0)
     med = float(np.median(passes[1:]))
+    # A larger open-shell system: (H2O)8 ... OH (211 AO), SCF seconds on a prepared plan.
+    from vibeqc_amd.basis import Atom
+    from vibeqc_amd.geometry import water_cluster
+
+    cluster = tuple(water_cluster(8, 6.0, seed=21)) + (Atom(8, (12.0, 0.0, 0.0)), Atom(1, (12.0, 0.0, 1.83)))
+    with Calculator(method="uhf").prepare(cluster, multiplicity=2) as plan:
+        plan.solve()
+        t0 = time.perf_counter()
+        big = plan.solve()
+        big_s = time.perf_counter() - t0
+        big_stats = plan.last_solve_stats()
     out = {
         "workload": "4 OH radicals, UHF doublets, cc-pVDZ (19 AO) / cc-pVDZ-JKFIT, energy + forces, one plan each",
         "systems": len(systems),
@@ -371,6 +383,13 @@ def _uhf_single(repeats=3):
         "systems_per_s": len(systems) / med,
         "energies": energies,
         "mean_iterations": float(np.mean(iterations)),
+        "water8_oh": {
+            "workload": "(H2O)8 ... OH doublet, cc-pVDZ (211 AO) / cc-pVDZ-JKFIT, UHF SCF on a prepared plan",
+            "scf_s": big_s,
+            "energy": big.energy,
+            "iterations": big.iterations,
+            "purified_densities": big_stats["purified_densities"],
+        },
     }
     _log(f"UHF single plans: {out}")
     return out
