@@ -373,26 +373,31 @@ int DfPlan::factorise_density(const double* D) {
   return static_cast<int>(pos.size()) | (r << 16);
 }
 
-JkResult DfPlan::evaluate(const std::vector<double>& density, std::vector<double>* gradient) {
-  converged_state_ = false;
-  VQC_HIP(hipSetDevice(device_));
-  const std::size_t nn = static_cast<std::size_t>(n_) * n_;
-  if (density.size() != nn) throw std::invalid_argument("density shape mismatch");
+std::vector<double> symmetrised_density(const std::vector<double>& d, int n) {
   double scale = 0.0, asym = 0.0;
-  for (int i = 0; i < n_; ++i)
-    for (int j = 0; j < n_; ++j) {
-      const double a = density[static_cast<std::size_t>(i) * n_ + j], b = density[static_cast<std::size_t>(j) * n_ + i];
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      const double a = d[static_cast<std::size_t>(i) * n + j], b = d[static_cast<std::size_t>(j) * n + i];
       if (!std::isfinite(a)) throw std::invalid_argument("density must be finite");
       scale = std::max(scale, std::abs(a));
       asym = std::max(asym, std::abs(a - b));
     }
   if (asym > 1e-10 * std::max(1.0, scale))
     throw std::invalid_argument("density-fitted exchange requires a symmetric density");
-  std::vector<double> sym(nn);
-  for (int i = 0; i < n_; ++i)
-    for (int j = 0; j < n_; ++j)
-      sym[static_cast<std::size_t>(i) * n_ + j] =
-          0.5 * (density[static_cast<std::size_t>(i) * n_ + j] + density[static_cast<std::size_t>(j) * n_ + i]);
+  std::vector<double> out(d.size());
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      out[static_cast<std::size_t>(i) * n + j] =
+          0.5 * (d[static_cast<std::size_t>(i) * n + j] + d[static_cast<std::size_t>(j) * n + i]);
+  return out;
+}
+
+JkResult DfPlan::evaluate(const std::vector<double>& density, std::vector<double>* gradient) {
+  converged_state_ = false;
+  VQC_HIP(hipSetDevice(device_));
+  const std::size_t nn = static_cast<std::size_t>(n_) * n_;
+  if (density.size() != nn) throw std::invalid_argument("density shape mismatch");
+  const std::vector<double> sym = symmetrised_density(density, n_);
   ensure_factor(1);
   D_.upload(sym.data(), nn, stream_);
   int packed = opts_.exchange ? factorise_density(D_.data()) : 0;

@@ -43,6 +43,11 @@ struct JkResult {
   std::vector<double> J, K;
 };
 
+// (D + D^T) / 2 of a finite n x n density that is symmetric to 1e-10 of its
+// largest element (density-fitted K is built from a factor of D); throws
+// std::invalid_argument otherwise.
+std::vector<double> symmetrised_density(const std::vector<double>& d, int n);
+
 // Unrestricted raw J (of the total density) and K of each spin density.
 struct UhfJk {
   std::vector<double> J, Ka, Kb;

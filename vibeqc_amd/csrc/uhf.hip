@@ -62,25 +62,6 @@ std::pair<int, int> spin_occupations(const System& sys) {
   return {static_cast<int>((electrons + excess) / 2), static_cast<int>((electrons - excess) / 2)};
 }
 
-std::vector<double> symmetric_copy(const std::vector<double>& d, int n) {
-  double scale = 0.0, asym = 0.0;
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) {
-      const double a = d[static_cast<std::size_t>(i) * n + j], b = d[static_cast<std::size_t>(j) * n + i];
-      if (!std::isfinite(a)) throw std::invalid_argument("density must be finite");
-      scale = std::max(scale, std::abs(a));
-      asym = std::max(asym, std::abs(a - b));
-    }
-  if (asym > 1e-10 * std::max(1.0, scale))
-    throw std::invalid_argument("density-fitted exchange requires a symmetric density");
-  std::vector<double> out(d.size());
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j)
-      out[static_cast<std::size_t>(i) * n + j] =
-          0.5 * (d[static_cast<std::size_t>(i) * n + j] + d[static_cast<std::size_t>(j) * n + i]);
-  return out;
-}
-
 }  // namespace
 
 void DfPlan::coulomb_only(Precision p, const double* D) {
@@ -129,7 +110,7 @@ UhfJk DfPlan::evaluate_unrestricted(const std::vector<double>& alpha, const std:
   VQC_HIP(hipSetDevice(device_));
   const std::size_t nn = static_cast<std::size_t>(n_) * n_;
   if (alpha.size() != nn || beta.size() != nn) throw std::invalid_argument("density shape mismatch");
-  const std::vector<double> a = symmetric_copy(alpha, n_), b = symmetric_copy(beta, n_);
+  const std::vector<double> a = symmetrised_density(alpha, n_), b = symmetrised_density(beta, n_);
   ensure_factor(1);
   DeviceArray<double> Da(nn), Db(nn);
   Da.upload(a.data(), nn, stream_);
@@ -278,7 +259,7 @@ ScfOutcome DfPlan::solve_unrestricted(const ScfOptions& o, const std::vector<dou
     if (initial_density->size() != 2 * nn) throw std::invalid_argument("unrestricted initial density must be 2 n^2");
     for (int s = 0; s < 2; ++s) {
       const std::vector<double> part(initial_density->begin() + s * nn, initial_density->begin() + (s + 1) * nn);
-      const std::vector<double> sym = symmetric_copy(part, n);
+      const std::vector<double> sym = symmetrised_density(part, n);
       copy_to_device(spin_block(D, s), sym.data(), nn * sizeof(double), stream_);
       copy_device(D_.data(), spin_block(D, s), nn, stream_);
       const int packed = factorise_density(D_.data());
