@@ -160,14 +160,15 @@ struct Vec16<float> {
   using type = float __attribute__((ext_vector_type(4)));
 };
 
-constexpr int kJThreads = 1024, kJInFlight = 8;
+constexpr int kJThreads = 1024, kJInFlight = 12;
 
 // J_p = sum_Q B[p][Q] rho_Q, one wavefront per packed row.  Each lane keeps
-// kJInFlight 16-byte loads of its row in flight before consuming them, and a
-// 1024-thread block shares one LDS copy of rho, so two blocks (32 waves) fit
-// on a CU: enough outstanding bytes to stream B at HBM rate.
+// kJInFlight 16-byte loads of its row in flight before consuming them (12
+// cover a 2432-wide FP32 row in one batch), and a 1024-thread block shares
+// one LDS copy of rho, so with up to 128 VGPRs per lane one block (16 waves)
+// per CU keeps enough bytes outstanding to stream B at HBM rate.
 template <class T>
-__global__ __launch_bounds__(kJThreads) __attribute__((amdgpu_waves_per_eu(8))) void coulomb_matrix_kernel(const T* __restrict__ B,
+__global__ __launch_bounds__(kJThreads) __attribute__((amdgpu_waves_per_eu(4))) void coulomb_matrix_kernel(const T* __restrict__ B,
                                                                   const double* __restrict__ rho,
                                                                   std::size_t npair, int ldq, int n,
                                                                   double* __restrict__ J) {
