@@ -212,3 +212,33 @@ def test_near_degenerate_radical_state_follows_the_seed(oracle):
     print(f"seeded E {r.energy:.12f} oracle {ref.energy:.12f} it={r.iterations}")
     assert abs(r.energy - ref.energy) < 1e-9
     assert np.abs(np.asarray(r.density) - ref.density).max() < 1e-6
+
+
+@pytest.mark.parametrize(
+    "name,atoms,charge,multiplicity",
+    [
+        ("o2_triplet", ((8, (0.0, 0.0, -1.14)), (8, (0.1, 0.05, 1.14))), 0, 3),
+        ("water_cation", ((8, (0.0, 0.0, 0.0)), (1, (0.0, 1.43, 1.11)), (1, (0.0, -1.43, 1.11))), 1, 2),
+    ],
+)
+def test_other_spin_states_match_oracle(oracle, name, atoms, charge, multiplicity):
+    """A triplet (n_alpha - n_beta = 2) and a cation doublet from cold starts:
+    energy to 1e-9 Eh and forces to 1e-7 Eh/bohr of the UHF oracle."""
+    from vibeqc_amd import Calculator
+    from vibeqc_amd.basis import Atom
+
+    atoms = tuple(Atom(z, xyz) for z, xyz in atoms)
+    case = molecule_case(name, atoms)
+    case.charge, case.multiplicity = charge, multiplicity
+    p = case.oracle_problem(oracle)
+    na, nb = oracle.spin_occupations(_electrons(case), multiplicity)
+    ref = oracle.df_uhf(p, na, nb, **TIGHT)
+    g = oracle.df_uhf_gradient(
+        case.z, case.xyz, case.oracle_shells(oracle), True, case.oracle_shells(oracle, True), True, p, ref, (na, nb)
+    )
+    r = Calculator(method="uhf", energy_tolerance=1e-12, density_tolerance=1e-10).singlepoint(
+        atoms, charge=charge, multiplicity=multiplicity
+    )
+    print(f"{name}: E {r.energy:.12f} oracle {ref.energy:.12f} dF {np.abs(np.asarray(r.forces) + g).max():.2e}")
+    assert abs(r.energy - ref.energy) < 1e-9
+    assert np.abs(np.asarray(r.forces) + g).max() < 1e-7
