@@ -242,3 +242,16 @@ def test_other_spin_states_match_oracle(oracle, name, atoms, charge, multiplicit
     print(f"{name}: E {r.energy:.12f} oracle {ref.energy:.12f} dF {np.abs(np.asarray(r.forces) + g).max():.2e}")
     assert abs(r.energy - ref.energy) < 1e-9
     assert np.abs(np.asarray(r.forces) + g).max() < 1e-7
+
+
+def test_mixed_precision_uhf_forces():
+    """BF16X3 iterations + FP64 refinement: energy within 1 uEh and forces
+    within 1e-6 Eh/bohr of the FP64 UHF (the forces come from FP64 builds)."""
+    from vibeqc_amd import Calculator
+
+    atoms = ((8, (0.0, 0.0, -1.14)), (8, (0.1, 0.05, 1.14)))
+    e64 = Calculator(method="uhf").singlepoint(atoms, multiplicity=3)
+    mixed = Calculator(method="uhf", precision="mixed").singlepoint(atoms, multiplicity=3)
+    print(f"dE {mixed.energy - e64.energy:.2e} dF {np.abs(np.asarray(mixed.forces) - np.asarray(e64.forces)).max():.2e}")
+    assert abs(mixed.energy - e64.energy) < 1e-6
+    assert np.abs(np.asarray(mixed.forces) - np.asarray(e64.forces)).max() < 1e-6
