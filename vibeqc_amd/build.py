@@ -69,12 +69,26 @@ def _compile(src: str) -> Path:
     return obj
 
 
-def build(jobs: int | None = None) -> Path:
-    """Compile all sources and link the shared library; returns its path."""
+def _stale(src: str) -> bool:
+    """True when the object of ``src`` is older than the source or any header."""
+    obj = BUILD / (Path(src).stem + ".o")
+    if not obj.exists():
+        return True
+    built = obj.stat().st_mtime
+    deps = [CSRC / src, *CSRC.glob("*.hpp"), *(ROOT / "include").rglob("*.h")]
+    return any(d.stat().st_mtime > built for d in deps)
+
+
+def build(jobs: int | None = None, changed_only: bool = False) -> Path:
+    """Compile the sources and link the shared library; returns its path.
+    With ``changed_only`` the objects that are newer than their source and
+    every header are reused."""
     BUILD.mkdir(exist_ok=True)
     jobs = jobs or min(len(SOURCES), max(1, min(16, os.cpu_count() or 1)))
+    todo = [s for s in SOURCES if not changed_only or _stale(s)]
     with ThreadPoolExecutor(max_workers=jobs) as pool:
-        objects = list(pool.map(_compile, SOURCES))
+        list(pool.map(_compile, todo))
+    objects = [BUILD / (Path(s).stem + ".o") for s in SOURCES]
     tmp = LIBRARY.with_suffix(".so.tmp")
     cmd = [
         _hipcc(),
@@ -96,5 +110,6 @@ def build(jobs: int | None = None) -> Path:
 
 
 if __name__ == "__main__":
-    path = build(int(sys.argv[1]) if len(sys.argv) > 1 else None)
+    args = [a for a in sys.argv[1:] if a != "--changed"]
+    path = build(int(args[0]) if args else None, changed_only="--changed" in sys.argv[1:])
     print(path)

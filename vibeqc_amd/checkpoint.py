@@ -31,6 +31,7 @@ import os
 import struct
 import tempfile
 import time
+from dataclasses import dataclass
 from pathlib import Path
 
 import numpy as np
@@ -144,187 +145,278 @@ def controls(calculator) -> dict:
     }
 
 
-# -- reading and validation ----------------------------------------------------
+# -- manifest schema -------------------------------------------------------------
+#
+# The manifest is checked in two layers.  First every record is conformed to a
+# field table: the record must carry exactly the listed fields and each field
+# must pass its predicate (each table entry names what is wrong when it does
+# not).  Then the rules that tie fields together run per item and over the blob
+# table.  Nothing is imported from a file that fails either layer.
 
 
-def _manifest_bytes(doc) -> bytes:
-    return json.dumps(doc, sort_keys=True, separators=(",", ":"), allow_nan=False).encode("utf-8")
+def _sha_hex(v) -> bool:
+    return isinstance(v, str) and len(v) == 64 and set(v) <= set("0123456789abcdef")
 
 
-def _unique(pairs):
-    out = {}
-    for k, v in pairs:
-        if k in out:
-            raise CheckpointError(f"duplicate manifest key: {k}")
-        out[k] = v
-    return out
+def _count(lo, hi):
+    return lambda v: type(v) is int and lo <= v <= hi
 
 
-def _reject_constant(name):
-    raise CheckpointError(f"nonfinite JSON: {name}")
+def _real(nonnegative=False):
+    return lambda v: type(v) in (int, float) and math.isfinite(v) and (v >= 0 or not nonnegative)
 
 
-def _int(value, name, lo, hi):
-    if type(value) is not int or not lo <= value <= hi:
-        raise CheckpointError(f"invalid {name}")
-    return value
+_ANY = lambda v: True  # noqa: E731
+_I32 = 2**31 - 1
+
+_RECORDS = {
+    "manifest": {
+        "schema": (lambda v: v == SCHEMA, "unsupported checkpoint schema"),
+        "schema_version": (lambda v: type(v) is int and v == 1, "unsupported checkpoint schema version"),
+        "array_format": (lambda v: v == ARRAY_FORMAT, "unsupported array format (dtype, endianness or order)"),
+        "items": (lambda v: isinstance(v, list) and 1 <= len(v) <= MAX_ITEMS, "item count outside 1..10000"),
+        "blobs": (lambda v: isinstance(v, list), "blob table is not a list"),
+    },
+    "item": {
+        "index": (lambda v: type(v) is int, "item index is not an integer"),
+        "atomic_numbers": (lambda v: isinstance(v, list) and len(v) > 0 and all(_count(1, 118)(z) for z in v),
+                           "ordered nuclei must be atomic numbers 1..118"),
+        "request_identity": (_sha_hex, "request identity is not a SHA-256 digest"),
+        "basis_metadata": (lambda v: isinstance(v, dict), "basis metadata is not an object"),
+        "model": (_ANY, ""),
+        "provider": (_ANY, ""),
+        "controls": (_ANY, ""),
+        "backend": (lambda v: v in ("cpu", "cuda"), "unknown source backend"),
+        "status": (_count(-1, _I32), "item status out of range"),
+        "seed": (_ANY, ""),
+    },
+    "controls": {
+        "max_iterations": (_count(0, _I32), "max_iterations out of range"),
+        "diis_history": (_count(0, _I32), "diis_history out of range"),
+        "energy_tolerance": (_real(True), "energy_tolerance must be finite and non-negative"),
+        "density_tolerance": (_real(True), "density_tolerance must be finite and non-negative"),
+        "screening_tolerance": (_real(True), "screening_tolerance must be finite and non-negative"),
+        "target_accuracy": (_ANY, ""),
+        "precision_policy": (_ANY, ""),
+        "runtime_policy": (_ANY, ""),
+    },
+    "model": {key: (_ANY, "") for key in MODEL_KEYS},
+    "seed": {
+        "density": (lambda v: isinstance(v, str), "density blob name is not a string"),
+        "coordinates": (lambda v: isinstance(v, str), "coordinates blob name is not a string"),
+        "nbf": (_count(1, _I32), "AO dimension out of range"),
+        "energy": (_real(), "source energy must be finite"),
+        "energy_change": (_real(), "source energy change must be finite"),
+        "density_rms": (_real(True), "source density residual must be finite and non-negative"),
+        "iterations": (_count(0, _I32), "source iteration count out of range"),
+        "converged": (lambda v: v is True, "schema 1 stores converged seeds only; this seed is not converged"),
+    },
+    "blob": {
+        "name": (lambda v: isinstance(v, str), "blob name is not a string"),
+        "shape": (lambda v: isinstance(v, list) and all(type(d) is int and d >= 0 for d in v), "blob dimensions are not non-negative integers"),
+        "offset": (lambda v: type(v) is int, "blob offset is not an integer"),
+        "bytes": (lambda v: type(v) is int, "blob byte count is not an integer"),
+        "sha256": (_sha_hex, "blob checksum is not a SHA-256 digest"),
+    },
+}
 
 
-def _fields(value, keys, name):
-    if not isinstance(value, dict) or set(value) != keys:
-        raise CheckpointError(f"unknown or missing required {name} fields")
+def _conform(kind: str, record, where: str) -> None:
+    table = _RECORDS[kind]
+    if not isinstance(record, dict):
+        raise CheckpointError(f"{where}: {kind} record is not an object")
+    if record.keys() != table.keys():
+        extra = sorted(set(record) - set(table))
+        absent = sorted(set(table) - set(record))
+        raise CheckpointError(f"{where}: unknown or missing {kind} fields (unknown {extra}, missing {absent})")
+    for key, (ok, problem) in table.items():
+        if not ok(record[key]):
+            raise CheckpointError(f"{where}: {problem}")
 
 
-def _is_hash(v) -> bool:
-    return isinstance(v, str) and len(v) == 64 and all(c in "0123456789abcdef" for c in v)
+def _strict_object(pairs):
+    seen = {}
+    for key, value in pairs:
+        if key in seen:
+            raise CheckpointError(f"duplicate manifest key {key!r}")
+        seen[key] = value
+    return seen
 
 
-def _number(v, name, *, nonnegative=False):
-    if type(v) not in (int, float) or not math.isfinite(v) or (nonnegative and v < 0):
-        raise CheckpointError(f"invalid {name}")
-    return float(v)
+def _no_constants(token):
+    raise CheckpointError(f"nonfinite JSON constant {token} in the manifest")
+
+
+def _seeded_item_rules(item, where, max_bytes):
+    """Cross-field rules of an item that carries a seed; returns the blob
+    shapes the seed needs, by name."""
+    model, seed, index = item["model"], item["seed"], item["index"]
+    _conform("model", model, where)
+    if canonical_hash({k: v for k, v in model.items() if k != "identity"}) != model["identity"]:
+        raise CheckpointError(f"{where}: model identity checksum mismatch")
+    if item["basis_metadata"]["orbital"]["mathematical_identity"] != model["basis_hash"]:
+        raise CheckpointError(f"{where}: orbital basis identity differs from the model's basis hash")
+    if sum(item["atomic_numbers"]) - model["charge"] != model["electron_count"]:
+        raise CheckpointError(f"{where}: electron count impossible for these nuclei and charge")
+    if item["provider"] != PROVIDER:
+        raise CheckpointError(f"{where}: unknown scientific provider {item['provider']!r}")
+    _conform("seed", seed, where)
+    n = seed["nbf"]
+    if n * n * 8 > max_bytes:
+        raise CheckpointError(f"{where}: AO dimension exceeds the byte limit")
+    spins = 2 if model["method"] == "uhf" else 1
+    wanted = {f"density_{index}": [spins, n, n], f"coordinates_{index}": [len(item["atomic_numbers"]), 3]}
+    if (seed["density"], seed["coordinates"]) != tuple(wanted):
+        raise CheckpointError(f"{where}: seed blob names must be density_{index} and coordinates_{index}")
+    return wanted
+
+
+def _blob_layout(blobs, wanted, max_bytes) -> int:
+    """Check the blob table against the shapes the seeds need; blobs follow
+    each other without gaps from payload offset 0.  Returns the payload length."""
+    end = 0
+    named = set()
+    for k, blob in enumerate(blobs):
+        where = f"blob {k}"
+        _conform("blob", blob, where)
+        name = blob["name"]
+        if name in named or name not in wanted:
+            raise CheckpointError(f"{where}: unexpected or duplicate blob {name!r}")
+        if blob["shape"] != wanted[name]:
+            raise CheckpointError(f"{where}: blob dimensions {blob['shape']} differ from the seed's {wanted[name]}")
+        nbytes = 8 * math.prod(blob["shape"])
+        if nbytes > max_bytes or blob["bytes"] != nbytes:
+            raise CheckpointError(f"{where}: blob byte count does not match its dimensions")
+        if blob["offset"] != end:
+            raise CheckpointError(f"{where}: blob offset {blob['offset']} is not the end of the previous blob ({end})")
+        named.add(name)
+        end += nbytes
+    if named != set(wanted):
+        raise CheckpointError(f"missing blobs: {sorted(set(wanted) - named)}")
+    return end
 
 
 def parse_manifest(raw: bytes, payload_bytes: int, max_bytes: int) -> dict:
-    """Validate a manifest against the schema and the payload length."""
+    """Decode and validate a manifest for a payload of ``payload_bytes``."""
     try:
-        doc = json.loads(raw, object_pairs_hook=_unique, parse_constant=_reject_constant)
+        doc = json.loads(raw, object_pairs_hook=_strict_object, parse_constant=_no_constants)
+    except CheckpointError:
+        raise
     except (UnicodeError, ValueError, RecursionError) as error:
-        if isinstance(error, CheckpointError):
-            raise
-        raise CheckpointError(f"invalid checkpoint manifest: {error}") from error
-    _fields(doc, {"schema", "schema_version", "array_format", "items", "blobs"}, "manifest")
-    if doc["schema"] != SCHEMA or type(doc["schema_version"]) is not int or doc["schema_version"] != 1:
-        raise CheckpointError("unsupported required checkpoint schema")
-    if doc["array_format"] != ARRAY_FORMAT:
-        raise CheckpointError("unsupported dtype/endian/array format")
-    items, blobs = doc["items"], doc["blobs"]
-    if not isinstance(items, list) or not 1 <= len(items) <= MAX_ITEMS:
-        raise CheckpointError("invalid checkpoint item count")
-    if not isinstance(blobs, list) or len(blobs) > 2 * len(items):
-        raise CheckpointError("invalid checkpoint blob count")
-    expected = {}
-    for index, item in enumerate(items):
-        _fields(item, ITEM_KEYS, "item")
-        if type(item["index"]) is not int or item["index"] != index:
-            raise CheckpointError("checkpoint item ordering mismatch")
-        numbers = item["atomic_numbers"]
-        if not isinstance(numbers, list) or not numbers or len(numbers) > max_bytes // 24:
-            raise CheckpointError("invalid ordered nuclei")
-        for z in numbers:
-            _int(z, "atomic number", 1, 118)
-        if not _is_hash(item["request_identity"]) or not isinstance(item["basis_metadata"], dict):
-            raise CheckpointError("missing prepared identity/provenance")
+        raise CheckpointError(f"manifest is not valid JSON: {error}") from error
+    _conform("manifest", doc, "manifest")
+    if len(doc["blobs"]) > 2 * len(doc["items"]):
+        raise CheckpointError("manifest: more blobs than the items' seeds can own")
+    wanted = {}
+    for position, item in enumerate(doc["items"]):
+        where = f"item {position}"
+        _conform("item", item, where)
+        if item["index"] != position:
+            raise CheckpointError(f"{where}: item ordering broken (index {item['index']})")
+        if len(item["atomic_numbers"]) > max_bytes // 24:
+            raise CheckpointError(f"{where}: too many nuclei for the byte limit")
         if item["basis_metadata"].get("model_identity") != item["request_identity"]:
-            raise CheckpointError("prepared identity/provenance mismatch")
-        _int(item["status"], "item status", -1, 2**31 - 1)
-        if item["backend"] not in ("cpu", "cuda"):
-            raise CheckpointError("invalid solver provenance")
-        c = item["controls"]
-        _fields(c, CONTROL_KEYS, "solver controls")
-        for name in ("max_iterations", "diis_history"):
-            _int(c[name], name, 0, 2**31 - 1)
-        for name in ("energy_tolerance", "density_tolerance", "screening_tolerance"):
-            _number(c[name], name, nonnegative=True)
-        seed = item["seed"]
-        if seed is None:
+            raise CheckpointError(f"{where}: prepared identity differs from the basis metadata's model identity")
+        _conform("controls", item["controls"], where)
+        if item["seed"] is None:
             if item["model"] is not None or item["provider"] is not None:
-                raise CheckpointError("no-state item carries reusable identity")
+                raise CheckpointError(f"{where}: no-state item carries a model or provider")
             continue
-        model = item["model"]
-        _fields(model, MODEL_KEYS, "model")
-        body = {k: v for k, v in model.items() if k != "identity"}
-        if canonical_hash(body) != model["identity"]:
-            raise CheckpointError("resolved-model identity checksum mismatch")
-        if item["basis_metadata"]["orbital"]["mathematical_identity"] != model["basis_hash"]:
-            raise CheckpointError("orbital basis provenance/model mismatch")
-        if sum(numbers) - model["charge"] != model["electron_count"]:
-            raise CheckpointError("impossible electron count for ordered nuclei")
-        if item["provider"] != PROVIDER:
-            raise CheckpointError("unsupported scientific provider identity")
-        _fields(seed, SEED_KEYS, "seed")
-        n = _int(seed["nbf"], "AO dimension", 1, math.isqrt(max_bytes // 8))
-        _int(seed["iterations"], "source iterations", 0, 2**31 - 1)
-        if seed["converged"] is not True:
-            raise CheckpointError("schema 1 requires a converged source seed")
-        for name in ("energy", "energy_change", "density_rms"):
-            _number(seed[name], "source diagnostics")
-        if seed["density_rms"] < 0:
-            raise CheckpointError("negative source residual")
-        spins = 2 if model["method"] == "uhf" else 1
-        for name, shape in (("density", [spins, n, n]), ("coordinates", [len(numbers), 3])):
-            if seed[name] != f"{name}_{index}":
-                raise CheckpointError("unexpected blob name")
-            expected[seed[name]] = shape
-    offset = 0
-    seen = set()
-    for blob in blobs:
-        _fields(blob, BLOB_KEYS, "blob")
-        name, shape = blob["name"], blob["shape"]
-        if not isinstance(shape, list) or any(type(d) is not int for d in shape):
-            raise CheckpointError("noninteger blob dimensions")
-        if not isinstance(name, str) or name in seen or name not in expected or shape != expected[name]:
-            raise CheckpointError("inconsistent or duplicate blob dimensions")
-        size = math.prod(shape) * 8
-        if size > max_bytes or type(blob["offset"]) is not int or blob["offset"] != offset or blob["bytes"] != size:
-            raise CheckpointError("inconsistent blob byte count/offset")
-        if not _is_hash(blob["sha256"]):
-            raise CheckpointError("invalid blob checksum")
-        seen.add(name)
-        offset += size
-    if seen != set(expected) or offset != payload_bytes:
-        raise CheckpointError("truncated, trailing or missing checkpoint blobs")
+        wanted.update(_seeded_item_rules(item, where, max_bytes))
+    declared = _blob_layout(doc["blobs"], wanted, max_bytes)
+    if payload_bytes < declared:
+        raise CheckpointError(f"payload truncated: {payload_bytes} bytes of {declared}")
+    if payload_bytes > declared:
+        raise CheckpointError(f"trailing bytes after the last blob: {payload_bytes - declared}")
     return doc
 
 
-def _read(path, max_bytes: int, retain: bool = True):
-    _int(max_bytes, "checkpoint byte limit", HEADER.size, 2**63 - 1)
-    arrays = {}
-    with open(path, "rb") as f:
-        size = os.fstat(f.fileno()).st_size
-        if size > max_bytes:
-            raise MemoryError("checkpoint file exceeds max_bytes")
-        header = f.read(HEADER.size)
-        if len(header) != HEADER.size:
-            raise CheckpointError("truncated checkpoint header")
-        magic, length, digest = HEADER.unpack(header)
-        if magic != MAGIC or length > min(MAX_MANIFEST, size - HEADER.size):
-            raise CheckpointError("invalid checkpoint header/schema or manifest length")
-        raw = f.read(length)
-        if hashlib.sha256(raw).digest() != digest:
-            raise CheckpointError("manifest checksum mismatch")
-        doc = parse_manifest(raw, size - HEADER.size - length, max_bytes)
-        for blob in doc["blobs"]:
-            h = hashlib.sha256()
-            data = bytearray() if retain else None
-            left = blob["bytes"]
-            while left:
-                chunk = f.read(min(left, 1 << 20))
-                if not chunk:
-                    raise CheckpointError("truncated checkpoint blob")
-                h.update(chunk)
-                if retain:
-                    data += chunk
-                left -= len(chunk)
-            if h.hexdigest() != blob["sha256"]:
-                raise CheckpointError(f"blob checksum mismatch: {blob['name']}")
-            if retain:
-                array = np.frombuffer(bytes(data), dtype="<f8").reshape(blob["shape"])
-                if not np.isfinite(array).all():
-                    raise CheckpointError("nonfinite checkpoint array")
-                arrays[blob["name"]] = array
-        if f.read(1):
-            raise CheckpointError("checkpoint changed while reading")
-    return doc, arrays, size
+# -- reading ---------------------------------------------------------------------
+
+
+class _CheckpointFile:
+    """Random-access view of one checkpoint file.  Blobs are read at their
+    absolute offsets (header + manifest + blob offset) with positional reads,
+    so verification does not depend on read order, and the file's size and
+    modification time are compared at the end to catch a concurrent writer."""
+
+    def __init__(self, path, max_bytes: int):
+        if type(max_bytes) is not int or max_bytes < HEADER.size:
+            raise CheckpointError("byte limit must be an integer of at least the header size")
+        self._fd = os.open(path, os.O_RDONLY)
+        try:
+            stat = os.fstat(self._fd)
+            self.size, self._stamp = stat.st_size, (stat.st_size, stat.st_mtime_ns)
+            if self.size > max_bytes:
+                raise MemoryError(f"checkpoint of {self.size} bytes exceeds max_bytes={max_bytes}")
+            magic, length, digest = HEADER.unpack(self._exact(0, HEADER.size, "header"))
+            if magic != MAGIC or length > min(MAX_MANIFEST, self.size - HEADER.size):
+                raise CheckpointError("not a checkpoint header (magic or manifest length)")
+            raw = self._exact(HEADER.size, length, "manifest")
+            if hashlib.sha256(raw).digest() != digest:
+                raise CheckpointError("manifest checksum mismatch")
+            self.payload_start = HEADER.size + length
+            self.manifest = parse_manifest(raw, self.size - self.payload_start, max_bytes)
+        except BaseException:
+            os.close(self._fd)
+            raise
+
+    def _exact(self, offset: int, length: int, what: str) -> bytes:
+        data = os.pread(self._fd, length, offset)
+        if len(data) != length:
+            raise CheckpointError(f"truncated checkpoint {what}")
+        return data
+
+    def blob(self, blob: dict, keep: bool):
+        """Verify one blob's checksum; with keep, return it as an array."""
+        digest = hashlib.sha256()
+        pieces = [] if keep else None
+        base = self.payload_start + blob["offset"]
+        for at in range(0, blob["bytes"], 1 << 22):
+            piece = self._exact(base + at, min(1 << 22, blob["bytes"] - at), "blob")
+            digest.update(piece)
+            if keep:
+                pieces.append(piece)
+        if digest.hexdigest() != blob["sha256"]:
+            raise CheckpointError(f"blob checksum mismatch for {blob['name']}")
+        if not keep:
+            return None
+        array = np.frombuffer(b"".join(pieces), dtype="<f8").reshape(blob["shape"])
+        if not np.all(np.isfinite(array)):
+            raise CheckpointError(f"nonfinite values in {blob['name']}")
+        return array
+
+    def close(self) -> None:
+        stat = os.fstat(self._fd)
+        os.close(self._fd)
+        if (stat.st_size, stat.st_mtime_ns) != self._stamp:
+            raise CheckpointError("checkpoint file changed while it was read")
+
+
+def _open_verified(path, max_bytes: int, keep_arrays: bool):
+    """(manifest, {blob name: array}, file size) after every check passed."""
+    view = _CheckpointFile(path, max_bytes)
+    try:
+        arrays = {b["name"]: view.blob(b, keep_arrays) for b in view.manifest["blobs"]}
+    except BaseException:
+        os.close(view._fd)
+        raise
+    view.close()
+    return view.manifest, (arrays if keep_arrays else {}), view.size
 
 
 def inspect_checkpoint(path, *, max_bytes: int = 256 << 20) -> dict:
     """Verify header, schema, lengths and every checksum; returns the manifest.
     Nothing is loaded into any batch and no device is touched."""
-    return _read(path, max_bytes, retain=False)[0]
+    return _open_verified(path, max_bytes, keep_arrays=False)[0]
 
 
 # -- save / load ---------------------------------------------------------------
+
+
+def _manifest_bytes(doc) -> bytes:
+    """Canonical manifest encoding: sorted keys, compact separators, no NaN."""
+    return json.dumps(doc, sort_keys=True, separators=(",", ":"), allow_nan=False).encode("utf-8")
 
 
 def _warm_state(batch, index: int):
@@ -458,77 +550,116 @@ def save_checkpoint(batch, path, *, max_bytes: int = 256 << 20) -> dict:
     }
 
 
+@dataclass
+class _SlotDecision:
+    """What loading does with one slot of the target batch."""
+
+    index: int
+    level: str
+    reason: str
+    source_status: int
+    source_identity: str | None
+    seed: dict | None = None  # the seed record when the slot held one
+    density: np.ndarray | None = None  # set when the seed is accepted
+    coordinates: np.ndarray | None = None
+
+    @property
+    def accepted(self) -> bool:
+        return self.density is not None
+
+    def report(self) -> dict:
+        return {
+            "index": self.index,
+            "compatibility": self.level,
+            "reason": self.reason,
+            "restored_fields": ["density"] if self.accepted else [],
+            "rejected_fields": ["density"] if self.seed is not None and not self.accepted else [],
+            "source_status": self.source_status,
+            "source_model_identity": self.source_identity,
+        }
+
+
+def _without(model: dict, *keys) -> dict:
+    return {k: v for k, v in model.items() if k not in keys}
+
+
+def _level_of(item, target_atoms, target_model, target_controls) -> tuple[str, str]:
+    """Compatibility level of a seeded item against its target slot: the first
+    mismatch found, from the strongest (different chemistry) to the weakest
+    (different geometry or numerical controls), decides."""
+    source = item["model"]
+    same_identity = _without(source, "identity", "geometry_hash") == _without(target_model, "identity", "geometry_hash")
+    basis_moved = source["basis_hash"] != target_model["basis_hash"] or source["representation"] != target_model["representation"]
+    checks = (
+        ([a.atomic_number for a in target_atoms] == item["atomic_numbers"], "incompatible", "ordered nuclei differ"),
+        (same_identity or not basis_moved, "transport_required", "basis/AO transport required"),
+        (same_identity, "incompatible", "method/core/spin/provider identity differs"),
+        (target_model["geometry_hash"] == source["geometry_hash"] and item["controls"] == target_controls,
+         "warm_start_compatible", "geometry or numerical controls differ"),
+    )
+    for holds, level, reason in checks:
+        if not holds:
+            return level, reason
+    return "exact_restart", "same scientific identity, geometry and controls"
+
+
+def _install(batch, decisions) -> None:
+    """Hand every accepted seed to the native batch in one call; the native
+    side validates them all before it replaces any."""
+    states = []
+    for d in decisions:
+        st = _native.descriptor(_native.HfWarmState)
+        if d.accepted:
+            st.density, st.coordinates = _native.pointer(d.density), _native.pointer(d.coordinates)
+            st.density_count, st.coordinate_count = d.density.size, d.coordinates.size
+            st.energy, st.energy_change = d.seed["energy"], d.seed["energy_change"]
+            st.density_rms, st.iterations, st.present = d.seed["density_rms"], d.seed["iterations"], 1
+        states.append(st)
+    table = (_native.HfWarmState * len(states))(*states)
+    status = batch._lib.generativeqc_batch_restore_hf_warm_states(batch._handle, table, len(states))
+    if status != 0:
+        raise CheckpointError(f"native checkpoint validation failed (status {status})")
+
+
 def load_checkpoint(batch, path, *, allow_warm: bool = False, strict: bool = True, max_bytes: int = 256 << 20) -> dict:
     """Validate the whole file, classify every slot against the target batch,
     then import the accepted seeds in one native call."""
-    from .basis import Atom
-
-    start = time.perf_counter()
+    began = time.perf_counter()
     batch._ensure_open()
-    doc, arrays, size = _read(path, max_bytes)
-    if len(doc["items"]) != len(batch._systems):
-        raise CheckpointError("checkpoint item count/order differs from target batch")
+    doc, arrays, size = _open_verified(path, max_bytes, keep_arrays=True)
+    targets = batch._systems
+    if len(doc["items"]) != len(targets):
+        raise CheckpointError(f"checkpoint holds {len(doc['items'])} slots, the target batch {len(targets)}")
     calc = batch._calculator
-    reports, states, keep = [], [], []
-    for i, item in enumerate(doc["items"]):
-        level, reason = "incompatible", "missing reusable state"
+    wanted_controls = controls(calc)
+    decisions = []
+    for item, atoms, charge, mult in zip(doc["items"], targets, batch._charges, batch._multiplicities):
         seed = item["seed"]
-        target_atoms = batch._systems[i]
-        st = _native.descriptor(_native.HfWarmState)
+        d = _SlotDecision(item["index"], "incompatible", "missing reusable state", item["status"],
+                          item["model"]["identity"] if seed is not None else None, seed)
         if seed is not None:
             coords = arrays[seed["coordinates"]]
             if geometry_hash(item["atomic_numbers"], coords) != item["model"]["geometry_hash"]:
-                raise CheckpointError("source coordinate/model identity mismatch")
-            target, _ = resolved_model(calc, target_atoms, batch._charges[i], batch._multiplicities[i])
-            src = {k: v for k, v in item["model"].items() if k not in ("identity", "geometry_hash")}
-            tgt = {k: v for k, v in target.items() if k not in ("identity", "geometry_hash")}
-            if [a.atomic_number for a in target_atoms] != item["atomic_numbers"]:
-                reason = "ordered nuclei differ"
-            elif src != tgt:
-                basis_differs = src["basis_hash"] != tgt["basis_hash"] or src["representation"] != tgt["representation"]
-                level = "transport_required" if basis_differs else "incompatible"
-                reason = "basis/AO transport required" if basis_differs else "method/core/spin/provider identity differs"
-            elif target["geometry_hash"] != item["model"]["geometry_hash"] or item["controls"] != controls(calc):
-                level, reason = "warm_start_compatible", "geometry or numerical controls differ"
-            else:
-                level, reason = "exact_restart", "same scientific identity, geometry and controls"
-        accepted = level == "exact_restart" or (allow_warm and level == "warm_start_compatible")
-        if accepted:
-            density = np.ascontiguousarray(arrays[seed["density"]].reshape(-1), dtype=np.float64)
-            coords = np.ascontiguousarray(arrays[seed["coordinates"]].reshape(-1), dtype=np.float64)
-            keep.extend((density, coords))
-            st.density, st.coordinates = _native.pointer(density), _native.pointer(coords)
-            st.density_count, st.coordinate_count = density.size, coords.size
-            st.energy, st.energy_change, st.density_rms = seed["energy"], seed["energy_change"], seed["density_rms"]
-            st.iterations, st.present = seed["iterations"], 1
-        states.append(st)
-        reports.append(
-            {
-                "index": i,
-                "compatibility": level,
-                "reason": reason,
-                "restored_fields": ["density"] if accepted else [],
-                "rejected_fields": [] if accepted or seed is None else ["density"],
-                "source_status": item["status"],
-                "source_model_identity": item["model"]["identity"] if seed else None,
-            }
-        )
-    rejected = [r for r in reports if not r["restored_fields"]]
-    if strict and rejected:
-        raise CheckpointError("checkpoint restore rejected: " + "; ".join(f"{r['index']}: {r['reason']}" for r in rejected))
-    accepted = {r["index"] for r in reports if r["restored_fields"]}
-    if accepted:
-        arr = (_native.HfWarmState * len(states))(*states)
-        status = batch._lib.generativeqc_batch_restore_hf_warm_states(batch._handle, arr, len(states))
-        if status != 0:
-            raise CheckpointError(f"native checkpoint validation failed (status {status})")
-    batch._restart_indices |= accepted
+                raise CheckpointError(f"item {d.index}: stored coordinates do not match the model's geometry hash")
+            target_model, _ = resolved_model(calc, atoms, charge, mult)
+            d.level, d.reason = _level_of(item, atoms, target_model, wanted_controls)
+            if d.level == "exact_restart" or (allow_warm and d.level == "warm_start_compatible"):
+                d.density = np.ascontiguousarray(arrays[seed["density"]], dtype=np.float64).reshape(-1)
+                d.coordinates = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
+        decisions.append(d)
+    refused = [d for d in decisions if not d.accepted]
+    if strict and refused:
+        raise CheckpointError("checkpoint restore rejected: " + "; ".join(f"{d.index}: {d.reason}" for d in refused))
+    restored = {d.index for d in decisions if d.accepted}
+    if restored:
+        _install(batch, decisions)
+    batch._restart_indices |= restored
     return {
         "schema_version": 1,
         "bytes": size,
         "manifest_identity": canonical_hash(doc),
-        "read_seconds": time.perf_counter() - start,
-        "items": reports,
+        "read_seconds": time.perf_counter() - began,
+        "items": [d.report() for d in decisions],
         "target_verification": "pending_execute",
     }
 

@@ -548,12 +548,17 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
 
   DeviceArray<double> X(nn), V(nn), W(nn), w(n), e(n), T1(nn), T2(nn), F(nn), Fe(nn), Dn(nn);
   DeviceArray<rocblas_int> info(1);
+  // scalars: [0..1] energy terms, [2] density RMS, [3] count of eigensolver
+  // failures (downloaded with the others, so no extra synchronisation).
   DeviceArray<double> scalars(4);
+  VQC_HIP(hipMemsetAsync(scalars.data(), 0, scalars.bytes(), stream_));
   auto eigh = [&](const double* A, double* vectors, double* values) {
     copy_device(vectors, A, nn, stream_);
     VQC_BLAS(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_lower, n, vectors, n, values, e.data(),
                               info.data()));
+    note_solver_failure(info.data(), scalars.data() + 3, stream_);
   };
+  auto eigh_failed = [](const double* host_scalars) { return host_scalars[3] != 0.0; };
   auto gemm = [&](rocblas_operation ta, rocblas_operation tb, int k, const double* A, int lda, const double* B,
                   int ldb, double* C) {
     VQC_BLAS(rocblas_dgemm(blas_, ta, tb, n, n, k, &one, A, lda, B, ldb, &zero, C, n));
@@ -562,8 +567,11 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   eigh(S_.data(), V.data(), w.data());
   {
     std::vector<double> hw(n);
+    double hs[4];
     w.download(hw.data(), n, stream_);
+    scalars.download(hs, 4, stream_);
     sync();
+    if (eigh_failed(hs)) throw NumericalFailure("overlap eigensolver did not converge");
     if (hw.front() < 1e-10) throw NumericalFailure("overlap matrix is singular or severely linearly dependent");
   }
   scale_columns_inv_sqrt(V.data(), w.data(), n, W.data(), stream_);
@@ -724,6 +732,7 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
       }
     }
     sync();
+    if (eigh_failed(hs)) throw NumericalFailure("Fock eigensolver did not converge");
     energy = 0.5 * (hs[0] + hs[1]) + enuc_;
     rms = hs[2];
     if (!std::isfinite(energy) || !std::isfinite(rms)) throw NumericalFailure("non-finite SCF iterate");
@@ -775,11 +784,12 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   assemble_fock(H_.data(), J_.data(), K_.data(), opts_.cj, opts_.ck, nn, F.data(), stream_);
   dot_pair(D_.data(), H_.data(), F.data(), nn, scalars.data(), stream_);
   out.fock_builds += 2;
-  double hs[2];
-  scalars.download(hs, 2, stream_);
+  double hs[4];
+  scalars.download(hs, 4, stream_);
   out.density.resize(nn);
   D_.download(out.density.data(), nn, stream_);
   sync();
+  if (eigh_failed(hs)) throw NumericalFailure("final Fock eigensolver did not converge");
   out.energy = 0.5 * (hs[0] + hs[1]) + enuc_;
   out.converged = std::isfinite(out.energy);
   if (out.converged) {

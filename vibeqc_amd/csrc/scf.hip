@@ -201,6 +201,10 @@ __global__ void finite_kernel(const double* __restrict__ x, std::size_t count, i
   if (bad) atomicOr(flag, 1);
 }
 
+__global__ void solver_failure_kernel(const rocblas_int* __restrict__ info, double* __restrict__ flag) {
+  if (threadIdx.x == 0 && info[0] != 0) flag[0] += 1.0;
+}
+
 __global__ void diagonal_kernel(const double* __restrict__ A, int n, double* __restrict__ diag) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) diag[i] = A[static_cast<std::size_t>(i) * n + i];
@@ -532,6 +536,11 @@ void pivoted_cholesky(const double* P, int n, int r, double* G, int* status, hip
   const std::size_t lds = static_cast<std::size_t>(n + r) * sizeof(double);
   hipLaunchKernelGGL(pivoted_cholesky_kernel, dim3(1), dim3(kCholThreads), lds, s, P, n, r, G, status);
   VQC_LAUNCH("pivoted_cholesky_kernel");
+}
+
+void note_solver_failure(const rocblas_int* info, double* flag, hipStream_t s) {
+  hipLaunchKernelGGL(solver_failure_kernel, dim3(1), dim3(64), 0, s, info, flag);
+  VQC_LAUNCH("solver_failure_kernel");
 }
 
 void extract_diagonal(const double* A, int n, double* diag, hipStream_t s) {

@@ -38,6 +38,10 @@ void factor_from_eigen(const double* V, const double* w, const int* idx, int n, 
                        hipStream_t s);
 // Non-finite detector: flag[0] |= any non-finite in x.
 void check_finite(const double* x, std::size_t count, int* flag, hipStream_t s);
+// Eigensolver status: flag[0] += 1 when info[0] != 0 (a rocSOLVER routine
+// did not converge), so a later download of flag reports the failure with no
+// extra synchronisation.
+void note_solver_failure(const rocblas_int* info, double* flag, hipStream_t s);
 // Symmetric copy from a column-major Cholesky factor: diag[i] = L[i + i*n].
 void extract_diagonal(const double* A, int n, double* diag, hipStream_t s);
 

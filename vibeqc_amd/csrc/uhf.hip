@@ -166,11 +166,15 @@ ScfOutcome DfPlan::solve_unrestricted(const ScfOptions& o, const std::vector<dou
   DeviceArray<double> X(nn), V(nn), w2(2 * static_cast<std::size_t>(n)), e(n), T1(nn), T2(nn);
   DeviceArray<double> D(2 * nn), Dn(2 * nn), F(2 * nn), Fe(2 * nn), Jt(nn), Ka(nn), W(2 * nn);
   DeviceArray<rocblas_int> info(1);
+  // scalars: [0..3] energy terms, [4] density RMS, [5] count of eigensolver
+  // failures (downloaded with the others).
   DeviceArray<double> scalars(8);
+  VQC_HIP(hipMemsetAsync(scalars.data(), 0, scalars.bytes(), stream_));
   auto eigh = [&](const double* A, double* vectors, double* values) {
     copy_device(vectors, A, nn, stream_);
     VQC_BLAS(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_lower, n, vectors, n, values, e.data(),
                               info.data()));
+    note_solver_failure(info.data(), scalars.data() + 5, stream_);
   };
   auto gemm = [&](rocblas_operation ta, rocblas_operation tb, int k, const double* A, int lda, const double* B,
                   int ldb, double* C) {
@@ -246,8 +250,11 @@ ScfOutcome DfPlan::solve_unrestricted(const ScfOptions& o, const std::vector<dou
   eigh(S_.data(), V.data(), w2.data());
   {
     std::vector<double> hw(n);
+    double hs[6];
     w2.download(hw.data(), n, stream_);
+    scalars.download(hs, 6, stream_);
     sync();
+    if (hs[5] != 0.0) throw NumericalFailure("overlap eigensolver did not converge");
     if (hw.front() < 1e-10) throw NumericalFailure("overlap matrix is singular or severely linearly dependent");
   }
   scale_columns_inv_sqrt(V.data(), w2.data(), n, T1.data(), stream_);
@@ -370,9 +377,10 @@ ScfOutcome DfPlan::solve_unrestricted(const ScfOptions& o, const std::vector<dou
     }
     from_orbitals = true;
     rms_difference(Dn.data(), D.data(), n2, scalars.data() + 4, stream_);
-    double hs[5];
-    scalars.download(hs, 5, stream_);
+    double hs[6];
+    scalars.download(hs, 6, stream_);
     sync();
+    if (hs[5] != 0.0) throw NumericalFailure("Fock eigensolver did not converge");
     energy = 0.5 * (hs[0] + hs[1] + hs[2] + hs[3]) + enuc_;
     rms = hs[4];
     if (!std::isfinite(energy) || !std::isfinite(rms)) throw NumericalFailure("non-finite SCF iterate");
@@ -423,11 +431,12 @@ ScfOutcome DfPlan::solve_unrestricted(const ScfOptions& o, const std::vector<dou
   build_focks(Precision::FP64, D, &W, F);
   energy_terms(D, F);
   out.fock_builds += 4;
-  double hs[4];
-  scalars.download(hs, 4, stream_);
+  double hs[6];
+  scalars.download(hs, 6, stream_);
   out.density.resize(n2);
   D.download(out.density.data(), n2, stream_);
   sync();
+  if (hs[5] != 0.0) throw NumericalFailure("final Fock eigensolver did not converge");
   out.energy = 0.5 * (hs[0] + hs[1] + hs[2] + hs[3]) + enuc_;
   out.converged = std::isfinite(out.energy);
   if (out.converged) {
