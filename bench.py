@@ -270,11 +270,9 @@ def _cpu_baseline(n, naux, nocc, fraction, seed=7):
 def _ragged_systems(count=512, seed=2):
     """Ragged batch: (H2O)n with n ~ U{1..16} from a seed, each in a cube of
     side 3 n^(1/3) A (rigid monomers, O-O >= 2.6 A)."""
-    from vibeqc_amd.geometry import water_cluster
+    from vibeqc_amd.geometry import ragged_water_batch
 
-    rng = np.random.default_rng(seed)
-    sizes = rng.integers(1, 17, size=count)
-    return [water_cluster(int(n), 3.0 * float(n) ** (1.0 / 3.0), seed=1000 + k) for k, n in enumerate(sizes)], sizes
+    return ragged_water_batch(count, seed)
 
 
 def _ragged_batch(count=512):

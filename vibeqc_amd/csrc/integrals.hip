@@ -232,7 +232,7 @@ void compute_one_electron(const System& sys, const DeviceBasis& basis, const dou
     args.T = T;
     args.V = V;
     args.n = static_cast<int>(basis.nfunctions);
-    args.sub = one_electron_lanes(args.natom);
+    args.sub = std::max(2, one_electron_lanes(args.natom));
     dispatch_1e(la, lb, args, s);
   }
   VQC_HIP(hipStreamSynchronize(s));

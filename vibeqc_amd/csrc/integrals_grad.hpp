@@ -291,213 +291,179 @@ struct OneElectronGradArgs {
   int sub;         // lanes per shell pair (1, 2, 4 or 8): they split the nuclei
 };
 
-// Two-index Hermite tables E[i][j][t] along one axis, i <= I, j <= J.
-template <int I, int J>
-__device__ __forceinline__ void hermite_ij(double pa, double pb, double h, double (&E)[I + 1][J + 1][I + J + 1]) {
-#pragma unroll
-  for (int i = 0; i <= I; ++i)
-#pragma unroll
-    for (int j = 0; j <= J; ++j)
-#pragma unroll
-      for (int t = 0; t <= I + J; ++t) E[i][j][t] = 0.0;
-  E[0][0][0] = 1.0;
-#pragma unroll
-  for (int i = 0; i < I; ++i)
-#pragma unroll
-    for (int t = 0; t <= i + 1; ++t) {
-      double v = 0.0;
-      if (t > 0) v += h * E[i][0][t - 1];
-      if (t <= i) v += pa * E[i][0][t];
-      if (t + 1 <= i) v += (t + 1) * E[i][0][t + 1];
-      E[i + 1][0][t] = v;
-    }
-#pragma unroll
-  for (int i = 0; i <= I; ++i)
-#pragma unroll
-    for (int j = 0; j < J; ++j)
-#pragma unroll
-      for (int t = 0; t <= i + j + 1; ++t) {
-        double v = 0.0;
-        if (t > 0) v += h * E[i][j][t - 1];
-        if (t <= i + j) v += pb * E[i][j][t];
-        if (t + 1 <= i + j) v += (t + 1) * E[i][j][t + 1];
-        E[i][j + 1][t] = v;
-      }
-}
-
-// One-electron force terms of one shell pair per lane:
+// One-electron force terms of one shell pair per group of a.sub lanes:
 //   sum D d(T + V)/dR - sum W dS/dR,
 // the overlap and kinetic parts through d/dB = -d/dA, the nuclear attraction
-// through d/dA and d/dB of each nucleus' term and its Hellmann-Feynman
-// derivative -(d/dA + d/dB) (s_integrals.cpp:1865-2060; hf_gradient.cpp:8-18).
+// through d/dA of each nucleus' term and its Hellmann-Feynman derivative
+// d/dC = -(d/dA + d/dB) (s_integrals.cpp:1865-2060; hf_gradient.cpp:8-18).
 //
-// a.sub consecutive lanes (an aligned group) share a shell pair: in round r
-// lane `part` of the group takes nucleus r * sub + part, so every lane works
-// in every round.  The Hellmann-Feynman term of a nucleus is summed over the
-// lanes of the wave with the same part, the pair's dA/dB over its group,
-// both by fixed butterflies; the overlap/kinetic terms count once (part 0).
+// The pair's Cartesian weights w = TA^T D TB (and the energy-weighted ones)
+// are formed once into LDS, one slot per pair of the wave.  Entry loops run
+// at run time and pick their Hermite factors by predicates, so no lane holds
+// a full table of them: per entry the three axes' value rows E^{ab}_t and
+// derivative rows 2a E^{a+1,b}_t - a E^{a-1,b}_t are rebuilt from the
+// recurrence.  In round r lane `part` of the group takes nucleus r sub + part,
+// so every lane works in every round; for that nucleus it forms dV/dA and the
+// Hellmann-Feynman term dV/dC = sum E_tuv (-R_{tuv + e_d}) and takes
+// dV/dB = -(dV/dA + dV/dC).  The Hellmann-Feynman term of a nucleus is summed
+// over the wave's lanes with the same part, the pair's dA/dB over its group,
+// both by fixed butterflies.
 template <int LA, int LB>
 __global__ __launch_bounds__(64) void one_electron_grad_kernel(const OneElectronGradArgs a) {
-  constexpr int NA = nc(LA), NB = nc(LB), L = LA + LB + 1;
-  constexpr int I = LA + 1, J = LB + 2;
+  constexpr int NA = nc(LA), NB = nc(LB), L = LA + LB + 1, NP = NA * NB;
+  extern __shared__ double pair_weights[];  // [64 / sub][2][NP]: wd, we per pair slot
   const int id = blockIdx.x * 64 + threadIdx.x;
   const int pair = id / a.sub, part = id % a.sub;
   const bool active = pair < a.ntask;
   const int2 task = a.tasks[active ? pair : 0];
   const DevShell A = a.shells[task.x];
   const DevShell B = a.shells[task.y];
-  double wd[NA][NB], we[NA][NB];
+  double* w = pair_weights + static_cast<std::size_t>(threadIdx.x / a.sub) * 2 * NP;
   {
     const double fac = active ? (task.x != task.y ? 2.0 : 1.0) : 0.0;
-    double pd[NA][NB], pe[NA][NB];
-#pragma unroll
-    for (int ma = 0; ma < NA; ++ma)
-#pragma unroll
-      for (int mb = 0; mb < NB; ++mb) {
-        double vd = 0.0, ve = 0.0;
-        if (ma < a.nfa && mb < a.nfb) {
+#pragma unroll 1
+    for (int e = part; e < NP; e += a.sub) {
+      const int i = e / NB, j = e % NB;
+      double vd = 0.0, ve = 0.0;
+      for (int ma = 0; ma < a.nfa; ++ma)
+        for (int mb = 0; mb < a.nfb; ++mb) {
+          const double t = a.ta[ma * NA + i] * a.tb[mb * NB + j];
           const std::size_t idx = static_cast<std::size_t>(A.fn_off + ma) * a.n + B.fn_off + mb;
-          vd = fac * a.dens[idx];
-          ve = fac * a.edens[idx];
+          vd += t * a.dens[idx];
+          ve += t * a.edens[idx];
         }
-        pd[ma][mb] = vd;
-        pe[ma][mb] = ve;
-      }
-#pragma unroll
-    for (int i = 0; i < NA; ++i)
-#pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        double vd = 0.0, ve = 0.0;
-#pragma unroll
-        for (int ma = 0; ma < NA; ++ma)
-#pragma unroll
-          for (int mb = 0; mb < NB; ++mb) {
-            const double t = a.ta[ma * NA + i] * a.tb[mb * NB + j];
-            vd += t * pd[ma][mb];
-            ve += t * pe[ma][mb];
-          }
-        wd[i][j] = vd;
-        we[i][j] = ve;
-      }
+      w[e] = fac * vd;
+      w[NP + e] = fac * ve;
+    }
   }
+  __syncthreads();
   const double ABx = A.x - B.x, ABy = A.y - B.y, ABz = A.z - B.z;
   const double AB2 = ABx * ABx + ABy * ABy + ABz * ABz;
   double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
-  // Overlap and kinetic: d/dA only.
+  // Overlap and kinetic: d/dA only, entries split over the group.
   for (int ia = 0; ia < A.nprim; ++ia) {
     const double ea = a.exps[A.prim_off + ia], ca = a.coefs[A.prim_off + ia];
     for (int ib = 0; ib < B.nprim; ++ib) {
       const double eb = a.exps[B.prim_off + ib], cb = a.coefs[B.prim_off + ib];
-      const double p = ea + eb, ip = 1.0 / p;
+      const double p = ea + eb, ip = 1.0 / p, h = 0.5 * ip;
       const double kab = ca * cb * exp(-ea * eb * ip * AB2);
       const double P[3] = {(ea * A.x + eb * B.x) * ip, (ea * A.y + eb * B.y) * ip, (ea * A.z + eb * B.z) * ip};
       const double s1 = sqrt(M_PI * ip);
-      double E[3][I + 1][J + 1][I + J + 1];
+      double ov[3][LA + 2][LB + 3];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) hermite_ij<I, J>(P[d] - (d == 0 ? A.x : d == 1 ? A.y : A.z),
-                                                   P[d] - (d == 0 ? B.x : d == 1 ? B.y : B.z), 0.5 * ip, E[d]);
+      for (int d = 0; d < 3; ++d)
+        overlap_1d<LA + 1, LB + 2>(P[d] - (d == 0 ? A.x : d == 1 ? A.y : A.z), P[d] - (d == 0 ? B.x : d == 1 ? B.y : B.z),
+                                   h, ov[d]);
+#pragma unroll 1
+      for (int e = part; e < NP; e += a.sub) {
+        const int i = e / NB, j = e % NB;
+        const int ai[3] = {cart_lx(LA, i), cart_ly(LA, i), cart_lz(LA, i)};
+        const int bj[3] = {cart_lx(LB, j), cart_ly(LB, j), cart_lz(LB, j)};
+        double sv[3], kv[3], sd[3], kd[3];
 #pragma unroll
-      for (int i = 0; i < NA; ++i)
+        for (int d = 0; d < 3; ++d) {
+          // Rows a - 1, a, a + 1 of the 1-D factors at columns b - 2, b, b + 2.
+          double f[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
 #pragma unroll
-        for (int j = 0; j < NB; ++j) {
-          const int ai[3] = {cart_lx(LA, i), cart_ly(LA, i), cart_lz(LA, i)};
-          const int bj[3] = {cart_lx(LB, j), cart_ly(LB, j), cart_lz(LB, j)};
-          double sv[3], kv[3], sd[3], kd[3];
+          for (int ii = 0; ii <= LA + 1; ++ii)
 #pragma unroll
-          for (int d = 0; d < 3; ++d) {
-            const int i0 = ai[d], j0 = bj[d];
-            auto kin = [&](int ii) {
-              double k = -2.0 * eb * eb * E[d][ii][j0 + 2][0] + eb * (2 * j0 + 1) * E[d][ii][j0][0];
-              if (j0 >= 2) k -= 0.5 * j0 * (j0 - 1) * E[d][ii][j0 - 2][0];
-              return k;
-            };
-            sv[d] = E[d][i0][j0][0] * s1;
-            kv[d] = kin(i0) * s1;
-            double sdd = 2.0 * ea * E[d][i0 + 1][j0][0];
-            double kdd = 2.0 * ea * kin(i0 + 1);
-            if (i0 > 0) {
-              sdd -= i0 * E[d][i0 - 1][j0][0];
-              kdd -= i0 * kin(i0 - 1);
+            for (int jj = 0; jj <= LB + 2; ++jj) {
+              const double x = ov[d][ii][jj];
+#pragma unroll
+              for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                  if (ii == ai[d] + r - 1 && jj == bj[d] + 2 * c - 2) f[r][c] = x;
             }
-            sd[d] = sdd * s1;
-            kd[d] = kdd * s1;
-          }
-#pragma unroll
-          for (int d = 0; d < 3; ++d) {
-            const int y = (d + 1) % 3, z = (d + 2) % 3;
-            const double dS = sd[d] * sv[y] * sv[z];
-            const double dT = kd[d] * sv[y] * sv[z] + sd[d] * kv[y] * sv[z] + sd[d] * sv[y] * kv[z];
-            gA[d] += kab * (wd[i][j] * dT - we[i][j] * dS);
-          }
+          const int b = bj[d];
+          auto kin = [&](int r) { return -2.0 * eb * eb * f[r][2] + eb * (2 * b + 1) * f[r][1] - 0.5 * b * (b - 1) * f[r][0]; };
+          sv[d] = f[1][1] * s1;
+          kv[d] = kin(1) * s1;
+          sd[d] = (2.0 * ea * f[2][1] - ai[d] * f[0][1]) * s1;
+          kd[d] = (2.0 * ea * kin(2) - ai[d] * kin(0)) * s1;
         }
+        const double wd = w[e], we = w[NP + e];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const int y = (d + 1) % 3, z = (d + 2) % 3;
+          const double dS = sd[d] * sv[y] * sv[z];
+          const double dT = kd[d] * sv[y] * sv[z] + sd[d] * kv[y] * sv[z] + sd[d] * sv[y] * kv[z];
+          gA[d] += kab * (wd * dT - we * dS);
+        }
+      }
     }
   }
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    if (part != 0) gA[d] = 0.0;
-    gB[d] = -gA[d];
-  }
+  for (int d = 0; d < 3; ++d) gB[d] = -gA[d];
   // Nuclear attraction, one nucleus per lane and round so the Hellmann-Feynman
   // term can be reduced across the wave (every lane runs the same rounds).
   for (int c0 = 0; c0 < a.natom; c0 += a.sub) {
     const int c = c0 + part;
     const bool has = c < a.natom;
     const double4 at = a.atoms[has ? c : 0];
-    double dA[3] = {0.0, 0.0, 0.0}, dB[3] = {0.0, 0.0, 0.0};
+    double dA[3] = {0.0, 0.0, 0.0}, dC[3] = {0.0, 0.0, 0.0};
     for (int ia = 0; ia < A.nprim; ++ia) {
       const double ea = a.exps[A.prim_off + ia], ca = a.coefs[A.prim_off + ia];
       for (int ib = 0; ib < B.nprim; ++ib) {
         const double eb = a.exps[B.prim_off + ib], cb = a.coefs[B.prim_off + ib];
-        const double p = ea + eb, ip = 1.0 / p;
+        const double p = ea + eb, ip = 1.0 / p, h = 0.5 * ip;
         const double kab = ca * cb * exp(-ea * eb * ip * AB2);
         const double P[3] = {(ea * A.x + eb * B.x) * ip, (ea * A.y + eb * B.y) * ip, (ea * A.z + eb * B.z) * ip};
-        double E[3][I + 1][J + 1][I + J + 1];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) hermite_ij<I, J>(P[d] - (d == 0 ? A.x : d == 1 ? A.y : A.z),
-                                                     P[d] - (d == 0 ? B.x : d == 1 ? B.y : B.z), 0.5 * ip, E[d]);
+        const double PA[3] = {P[0] - A.x, P[1] - A.y, P[2] - A.z};
+        const double PB[3] = {P[0] - B.x, P[1] - B.y, P[2] - B.z};
         const double X = P[0] - at.x, Y = P[1] - at.y, Z = P[2] - at.z;
         double F[L + 1];
         boys_eval<L>(p * (X * X + Y * Y + Z * Z), a.boys, F);
         double R[ntuv(L)];
         hermite_r<L>(p, X, Y, Z, F, R);
-        const double pref = -at.w * 2.0 * M_PI * ip * kab;
+        const double pref = has ? -at.w * 2.0 * M_PI * ip * kab : 0.0;
+#pragma unroll 1
+        for (int e = 0; e < NP; ++e) {
+          const int i = e / NB, j = e % NB;
+          const int ai[3] = {cart_lx(LA, i), cart_ly(LA, i), cart_lz(LA, i)};
+          const int bj[3] = {cart_lx(LB, j), cart_ly(LB, j), cart_lz(LB, j)};
+          // Value rows E^{ab}_t and A-derivative rows of the three axes.
+          double val[3][L + 1], der[3][L + 1];
 #pragma unroll
-        for (int i = 0; i < NA; ++i)
+          for (int x = 0; x < 3; ++x) {
+            double up[L + 1], dn[L + 1];
+            hermite_1d<L>(ai[x], bj[x], PA[x], PB[x], h, val[x]);
+            hermite_1d<L>(ai[x] + 1, bj[x], PA[x], PB[x], h, up);
+            hermite_1d<L>(ai[x] > 0 ? ai[x] - 1 : 0, bj[x], PA[x], PB[x], h, dn);
 #pragma unroll
-          for (int j = 0; j < NB; ++j) {
-            const int ai[3] = {cart_lx(LA, i), cart_ly(LA, i), cart_lz(LA, i)};
-            const int bj[3] = {cart_lx(LB, j), cart_ly(LB, j), cart_lz(LB, j)};
-            const double w = has ? pref * wd[i][j] : 0.0;
-#pragma unroll
-            for (int which = 0; which < 2; ++which)
-#pragma unroll
-              for (int d = 0; d < 3; ++d) {
-                // Coefficient along axis x: derivative along d, value elsewhere.
-                auto coef = [&](int x, int t) {
-                  const int i0 = ai[x], j0 = bj[x];
-                  if (x != d) return E[x][i0][j0][t];
-                  if (which == 0) return 2.0 * ea * E[x][i0 + 1][j0][t] - (i0 > 0 ? i0 * E[x][i0 - 1][j0][t] : 0.0);
-                  return 2.0 * eb * E[x][i0][j0 + 1][t] - (j0 > 0 ? j0 * E[x][i0][j0 - 1][t] : 0.0);
-                };
-                double s = 0.0;
-#pragma unroll
-                for (int t = 0; t <= ai[0] + bj[0] + (d == 0); ++t)
-#pragma unroll
-                  for (int u = 0; u <= ai[1] + bj[1] + (d == 1); ++u)
-#pragma unroll
-                    for (int v = 0; v <= ai[2] + bj[2] + (d == 2); ++v)
-                      s += coef(0, t) * coef(1, u) * coef(2, v) * R[tuv(t, u, v)];
-                if (which == 0) dA[d] += w * s;
-                else dB[d] += w * s;
-              }
+            for (int t = 0; t <= L; ++t) der[x][t] = 2.0 * ea * up[t] - (ai[x] > 0 ? ai[x] * dn[t] : 0.0);
           }
+          const int n0 = ai[0] + bj[0], n1 = ai[1] + bj[1], n2 = ai[2] + bj[2];
+          const double wt = pref * w[e];
+          double sa[3] = {0.0, 0.0, 0.0}, sc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+          for (int t = 0; t <= L; ++t)
+#pragma unroll
+            for (int u = 0; u <= L - t; ++u)
+#pragma unroll
+              for (int v = 0; v <= L - t - u; ++v) {
+                const double r = R[tuv(t, u, v)];
+                if (t <= n0 + 1 && u <= n1 && v <= n2) sa[0] += der[0][t] * val[1][u] * val[2][v] * r;
+                if (t <= n0 && u <= n1 + 1 && v <= n2) sa[1] += val[0][t] * der[1][u] * val[2][v] * r;
+                if (t <= n0 && u <= n1 && v <= n2 + 1) sa[2] += val[0][t] * val[1][u] * der[2][v] * r;
+                // dV/dC: the value coefficients against R shifted by one along each axis.
+                if (t >= 1 && t - 1 <= n0 && u <= n1 && v <= n2) sc[0] -= val[0][t - 1] * val[1][u] * val[2][v] * r;
+                if (u >= 1 && t <= n0 && u - 1 <= n1 && v <= n2) sc[1] -= val[0][t] * val[1][u - 1] * val[2][v] * r;
+                if (v >= 1 && t <= n0 && u <= n1 && v - 1 <= n2) sc[2] -= val[0][t] * val[1][u] * val[2][v - 1] * r;
+              }
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            dA[d] += wt * sa[d];
+            dC[d] += wt * sc[d];
+          }
+        }
       }
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       gA[d] += dA[d];
-      gB[d] += dB[d];
-      double hf = -(dA[d] + dB[d]);
+      gB[d] -= dA[d] + dC[d];
+      double hf = dC[d];
       for (int off = 32; off >= a.sub; off >>= 1) hf += __shfl_xor(hf, off);
       if (threadIdx.x < a.sub && has) a.out_c[(static_cast<std::size_t>(blockIdx.x) * a.natom + c) * 3 + d] = hf;
     }
@@ -553,8 +519,9 @@ void launch_1e_grad(const OneElectronGradArgs& args, hipStream_t s) {
   if (args.ntask == 0) return;
   if (args.sub < 1 || args.sub > 8 || (args.sub & (args.sub - 1)) != 0)
     throw std::invalid_argument("one-electron lanes per pair must be 1, 2, 4 or 8");
+  const std::size_t lds = static_cast<std::size_t>(64 / args.sub) * 2 * nc(LA) * nc(LB) * sizeof(double);
   hipLaunchKernelGGL((one_electron_grad_kernel<LA, LB>), dim3(one_electron_grad_blocks(args.ntask, args.sub)),
-                     dim3(64), 0, s, args);
+                     dim3(64), lds, s, args);
   VQC_LAUNCH("one_electron_grad_kernel");
 }
 

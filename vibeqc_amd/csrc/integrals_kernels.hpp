@@ -361,25 +361,83 @@ inline int one_electron_lanes(int natom) {
   return sub;
 }
 
+// Overlap factors E^{ij}_0 along one axis for i <= I, j <= J by the
+// Obara-Saika recurrences S(i+1,j) = PA S(i,j) + h (i S(i-1,j) + j S(i,j-1))
+// and S(0,j+1) = PB S(0,j) + h j S(0,j-1), h = 1/(2p), S(0,0) = 1.
+template <int I, int J>
+__device__ __forceinline__ void overlap_1d(double pa, double pb, double h, double (&s)[I + 1][J + 1]) {
+#pragma unroll
+  for (int i = 0; i <= I; ++i)
+#pragma unroll
+    for (int j = 0; j <= J; ++j) {
+      double v = 1.0;
+      if (i > 0) {
+        v = pa * s[i - 1][j];
+        if (i > 1) v += h * (i - 1) * s[i - 2][j];
+        if (j > 0) v += h * j * s[i - 1][j - 1];
+      } else if (j > 0) {
+        v = pb * s[0][j - 1];
+        if (j > 1) v += h * (j - 1) * s[0][j - 2];
+      }
+      s[i][j] = v;
+    }
+}
+
+// Hermite expansion coefficients E^{ij}_t (t <= i + j) of one axis of a
+// Gaussian product, for exponents i, j that are compile-time constants after
+// unrolling: E^{i+1,j}_t = h E^{ij}_{t-1} + PA E^{ij}_t + (t+1) E^{ij}_{t+1}
+// from E^{00}_0 = 1, then the same transfer in j with PB.
+template <int T>
+__device__ __forceinline__ void hermite_1d(int i, int j, double pa, double pb, double h, double (&e)[T + 1]) {
+#pragma unroll
+  for (int t = 0; t <= T; ++t) e[t] = t == 0 ? 1.0 : 0.0;
+#pragma unroll
+  for (int k = 0; k < T; ++k) {
+    if (k >= i + j) break;
+    const int deg = k;  // degree of the current row
+    const double f = k < i ? pa : pb;
+    double nxt[T + 1];
+#pragma unroll
+    for (int t = 0; t <= T; ++t) {
+      double v = 0.0;
+      if (t <= deg + 1) {
+        if (t > 0) v += h * e[t - 1];
+        if (t <= deg) v += f * e[t];
+        if (t + 1 <= deg) v += (t + 1) * e[t + 1];
+      }
+      nxt[t] = v;
+    }
+#pragma unroll
+    for (int t = 0; t <= T; ++t) e[t] = nxt[t];
+  }
+}
+
+// Overlap, kinetic and nuclear-attraction integrals of one shell pair per
+// group of a.sub consecutive lanes (s_integrals.cpp:1865-2060).
+//
+// Per primitive pair the group's lanes split the Cartesian entries (i, j)
+// of S and T (entry e belongs to lane e % sub) and the nuclei of V.  V is
+// contracted once per primitive pair: every lane sums the Coulomb Hermite
+// integrals of its nuclei, R~_tuv = sum_C -Z_C R_tuv(P - C), the group adds
+// its partial sums by a fixed butterfly, and the owner of each entry forms
+// sum_tuv E^x_t E^y_u E^z_v R~_tuv.  The Cartesian accumulators live in LDS,
+// one slot per pair of the wave, so no lane keeps the whole S/T/V block in
+// registers.  Lanes past the last pair compute pair 0 into their own slot and
+// write nothing, so every lane reaches the shuffles.
 template <int LA, int LB>
 __global__ __launch_bounds__(64) void one_electron_kernel(const OneElectronArgs a) {
-  constexpr int NA = nc(LA), NB = nc(LB), L = LA + LB;
-  constexpr int JB = LB + 2, TD = LA + JB + 1;
-  // a.sub consecutive lanes (one aligned group of a wave) share a shell pair:
-  // each takes every a.sub-th nucleus of V, and the group's V is summed by a
-  // fixed butterfly.  Lanes past the last pair compute pair 0 and write
-  // nothing, so every lane reaches the shuffles.
+  constexpr int NA = nc(LA), NB = nc(LB), L = LA + LB, NP = NA * NB;
+  extern __shared__ double cart_acc[];  // [64 / sub][3][NP]: S, T, V per pair slot
   const int id = blockIdx.x * 64 + threadIdx.x;
   const int pair = id / a.sub, part = id % a.sub;
   const bool active = pair < a.ntask;
   const int2 task = a.tasks[active ? pair : 0];
   const DevShell A = a.shells[task.x];
   const DevShell B = a.shells[task.y];
-  double s[NA][NB], k[NA][NB], v[NA][NB];
+  double* slot = cart_acc + static_cast<std::size_t>(threadIdx.x / a.sub) * 3 * NP;
 #pragma unroll
-  for (int i = 0; i < NA; ++i)
-#pragma unroll
-    for (int j = 0; j < NB; ++j) s[i][j] = k[i][j] = v[i][j] = 0.0;
+  for (int e = 0; e < NP; ++e)
+    if (e % a.sub == part) slot[e] = slot[NP + e] = slot[2 * NP + e] = 0.0;
   const double ABx = A.x - B.x, ABy = A.y - B.y, ABz = A.z - B.z;
   const double AB2 = ABx * ABx + ABy * ABy + ABz * ABz;
   for (int ia = 0; ia < A.nprim; ++ia) {
@@ -391,58 +449,45 @@ __global__ __launch_bounds__(64) void one_electron_kernel(const OneElectronArgs 
       const double P[3] = {(ea * A.x + eb * B.x) * ip, (ea * A.y + eb * B.y) * ip, (ea * A.z + eb * B.z) * ip};
       const double PA[3] = {P[0] - A.x, P[1] - A.y, P[2] - A.z};
       const double PB[3] = {P[0] - B.x, P[1] - B.y, P[2] - B.z};
-      // Two-index Hermite tables E[d][i][j][t] for i <= LA, j <= LB + 2.
-      double E[3][LA + 1][JB + 1][TD];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-#pragma unroll
-        for (int i = 0; i <= LA; ++i)
-#pragma unroll
-          for (int j = 0; j <= JB; ++j)
-#pragma unroll
-            for (int t = 0; t < TD; ++t) E[d][i][j][t] = 0.0;
-        E[d][0][0][0] = 1.0;
-#pragma unroll
-        for (int i = 0; i < LA; ++i)
-#pragma unroll
-          for (int t = 0; t <= i + 1; ++t) {
-            double val = 0.0;
-            if (t > 0) val += h * E[d][i][0][t - 1];
-            if (t <= i) val += PA[d] * E[d][i][0][t];
-            if (t + 1 <= i) val += (t + 1) * E[d][i][0][t + 1];
-            E[d][i + 1][0][t] = val;
-          }
-#pragma unroll
-        for (int i = 0; i <= LA; ++i)
-#pragma unroll
-          for (int j = 0; j < JB; ++j)
-#pragma unroll
-            for (int t = 0; t <= i + j + 1; ++t) {
-              double val = 0.0;
-              if (t > 0) val += h * E[d][i][j][t - 1];
-              if (t <= i + j) val += PB[d] * E[d][i][j][t];
-              if (t + 1 <= i + j) val += (t + 1) * E[d][i][j][t + 1];
-              E[d][i][j + 1][t] = val;
-            }
-      }
       const double s1 = sqrt(M_PI * ip);
+      {
+        // S and T: separable, from 1-D overlap factors up to j + 2.  The
+        // entry loop runs at run time (its exponents select factors by
+        // predicates, so every register index stays a constant).
+        double ov[3][LA + 1][LB + 3];
 #pragma unroll
-      for (int i = 0; i < NA; ++i)
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
+        for (int d = 0; d < 3; ++d) overlap_1d<LA, LB + 2>(PA[d], PB[d], h, ov[d]);
+#pragma unroll 1
+        for (int e = part; e < NP; e += a.sub) {
+          const int i = e / NB, j = e % NB;
           const int ai[3] = {cart_lx(LA, i), cart_ly(LA, i), cart_lz(LA, i)};
           const int bj[3] = {cart_lx(LB, j), cart_ly(LB, j), cart_lz(LB, j)};
           double sd[3], kd[3];
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
-            sd[d] = E[d][ai[d]][bj[d]][0] * s1;
-            double kv = -2.0 * eb * eb * E[d][ai[d]][bj[d] + 2][0] * s1 + eb * (2 * bj[d] + 1) * sd[d];
-            if (bj[d] >= 2) kv -= 0.5 * bj[d] * (bj[d] - 1) * E[d][ai[d]][bj[d] - 2][0] * s1;
-            kd[d] = kv;
+            double s0 = 0.0, sp = 0.0, sm = 0.0;
+#pragma unroll
+            for (int ii = 0; ii <= LA; ++ii)
+#pragma unroll
+              for (int jj = 0; jj <= LB + 2; ++jj) {
+                const double x = ov[d][ii][jj];
+                if (ii == ai[d]) {
+                  s0 = jj == bj[d] ? x : s0;
+                  sp = jj == bj[d] + 2 ? x : sp;
+                  sm = jj + 2 == bj[d] ? x : sm;
+                }
+              }
+            sd[d] = s0 * s1;
+            kd[d] = (-2.0 * eb * eb * sp + eb * (2 * bj[d] + 1) * s0 - 0.5 * bj[d] * (bj[d] - 1) * sm) * s1;
           }
-          s[i][j] += kab * sd[0] * sd[1] * sd[2];
-          k[i][j] += kab * (kd[0] * sd[1] * sd[2] + sd[0] * kd[1] * sd[2] + sd[0] * sd[1] * kd[2]);
+          slot[e] += kab * sd[0] * sd[1] * sd[2];
+          slot[NP + e] += kab * (kd[0] * sd[1] * sd[2] + sd[0] * kd[1] * sd[2] + sd[0] * sd[1] * kd[2]);
         }
+      }
+      // V: nuclear Hermite sums of this lane's nuclei, then the group's total.
+      double rs[ntuv(L)];
+#pragma unroll
+      for (int k = 0; k < ntuv(L); ++k) rs[k] = 0.0;
       for (int c = part; c < a.natom; c += a.sub) {
         const double4 at = a.atoms[c];
         const double X = P[0] - at.x, Y = P[1] - at.y, Z = P[2] - at.z;
@@ -450,53 +495,61 @@ __global__ __launch_bounds__(64) void one_electron_kernel(const OneElectronArgs 
         boys_eval<L>(p * (X * X + Y * Y + Z * Z), a.boys, F);
         double R[ntuv(L)];
         hermite_r<L>(p, X, Y, Z, F, R);
-        const double pref = -at.w * 2.0 * M_PI * ip * kab;
 #pragma unroll
-        for (int i = 0; i < NA; ++i)
+        for (int k = 0; k < ntuv(L); ++k) rs[k] -= at.w * R[k];
+      }
+      for (int off = 1; off < a.sub; off <<= 1)
 #pragma unroll
-          for (int j = 0; j < NB; ++j) {
-            const int ai[3] = {cart_lx(LA, i), cart_ly(LA, i), cart_lz(LA, i)};
-            const int bj[3] = {cart_lx(LB, j), cart_ly(LB, j), cart_lz(LB, j)};
-            double acc = 0.0;
+        for (int k = 0; k < ntuv(L); ++k) rs[k] += __shfl_xor(rs[k], off);
+      const double pref = 2.0 * M_PI * ip * kab;
+#pragma unroll 1
+      for (int e = part; e < NP; e += a.sub) {
+        const int i = e / NB, j = e % NB;
+        const int ai[3] = {cart_lx(LA, i), cart_ly(LA, i), cart_lz(LA, i)};
+        const int bj[3] = {cart_lx(LB, j), cart_ly(LB, j), cart_lz(LB, j)};
+        double ex[L + 1], ey[L + 1], ez[L + 1];
+        hermite_1d<L>(ai[0], bj[0], PA[0], PB[0], h, ex);
+        hermite_1d<L>(ai[1], bj[1], PA[1], PB[1], h, ey);
+        hermite_1d<L>(ai[2], bj[2], PA[2], PB[2], h, ez);
+        const int nx = ai[0] + bj[0], ny = ai[1] + bj[1], nz = ai[2] + bj[2];
+        double val = 0.0;
 #pragma unroll
-            for (int t = 0; t <= ai[0] + bj[0]; ++t)
+        for (int t = 0; t <= L; ++t)
 #pragma unroll
-              for (int u = 0; u <= ai[1] + bj[1]; ++u)
+          for (int u = 0; u <= L - t; ++u)
 #pragma unroll
-                for (int w = 0; w <= ai[2] + bj[2]; ++w)
-                  acc += E[0][ai[0]][bj[0]][t] * E[1][ai[1]][bj[1]][u] * E[2][ai[2]][bj[2]][w] * R[tuv(t, u, w)];
-            v[i][j] += pref * acc;
-          }
+            for (int v = 0; v <= L - t - u; ++v)
+              if (t <= nx && u <= ny && v <= nz) val += ex[t] * ey[u] * ez[v] * rs[tuv(t, u, v)];
+        slot[2 * NP + e] += pref * val;
       }
     }
   }
-  for (int off = 1; off < a.sub; off <<= 1)
+  // The group's entries were written by its own lanes (one wave); publish
+  // them to the whole group before the transform.
+  __syncthreads();
+  if (!active) return;
+  // Public AOs: out = TA x TB^T, written to both triangles; the group's lanes
+  // split the public entries.
+  for (int m = part; m < a.nfa * a.nfb; m += a.sub) {
+    const int ma = m / a.nfb, mb = m % a.nfb;
+    double os = 0.0, ok = 0.0, ov = 0.0;
 #pragma unroll
     for (int i = 0; i < NA; ++i)
 #pragma unroll
-      for (int j = 0; j < NB; ++j) v[i][j] += __shfl_xor(v[i][j], off);
-  if (!active || part != 0) return;
-  // Public AOs: out = TA x TB^T, written to both triangles.
-  for (int ma = 0; ma < a.nfa; ++ma)
-    for (int mb = 0; mb < a.nfb; ++mb) {
-      double os = 0.0, ok = 0.0, ov = 0.0;
-#pragma unroll
-      for (int i = 0; i < NA; ++i)
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-          const double w = a.ta[ma * NA + i] * a.tb[mb * NB + j];
-          os += w * s[i][j];
-          ok += w * k[i][j];
-          ov += w * v[i][j];
-        }
-      const std::size_t mu = A.fn_off + ma, nu = B.fn_off + mb;
-      a.S[mu * a.n + nu] = os;
-      a.S[nu * a.n + mu] = os;
-      a.T[mu * a.n + nu] = ok;
-      a.T[nu * a.n + mu] = ok;
-      a.V[mu * a.n + nu] = ov;
-      a.V[nu * a.n + mu] = ov;
-    }
+      for (int j = 0; j < NB; ++j) {
+        const double w = a.ta[ma * NA + i] * a.tb[mb * NB + j];
+        os += w * slot[i * NB + j];
+        ok += w * slot[NP + i * NB + j];
+        ov += w * slot[2 * NP + i * NB + j];
+      }
+    const std::size_t mu = A.fn_off + ma, nu = B.fn_off + mb;
+    a.S[mu * a.n + nu] = os;
+    a.S[nu * a.n + mu] = os;
+    a.T[mu * a.n + nu] = ok;
+    a.T[nu * a.n + mu] = ok;
+    a.V[mu * a.n + nu] = ov;
+    a.V[nu * a.n + mu] = ov;
+  }
 }
 
 template <int LA, int LB, int LC, bool METRIC, bool CART>
@@ -530,7 +583,8 @@ void launch_1e(const OneElectronArgs& args, hipStream_t s) {
   if (args.sub < 1 || args.sub > 8 || (args.sub & (args.sub - 1)) != 0)
     throw std::invalid_argument("one-electron lanes per pair must be 1, 2, 4 or 8");
   const std::size_t lanes = static_cast<std::size_t>(args.ntask) * args.sub;
-  hipLaunchKernelGGL((one_electron_kernel<LA, LB>), dim3(static_cast<unsigned>((lanes + 63) / 64)), dim3(64), 0, s,
+  const std::size_t lds = static_cast<std::size_t>(64 / args.sub) * 3 * nc(LA) * nc(LB) * sizeof(double);
+  hipLaunchKernelGGL((one_electron_kernel<LA, LB>), dim3(static_cast<unsigned>((lanes + 63) / 64)), dim3(64), lds, s,
                      args);
   VQC_LAUNCH("one_electron_kernel");
 }

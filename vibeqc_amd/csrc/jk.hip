@@ -223,8 +223,8 @@ constexpr int kUTileQ = 256;
 
 template <class T, int NIW, int WQ>
 __global__ __launch_bounds__(128 * WQ) void occupied_u_kernel(const T* __restrict__ B, const T* __restrict__ C,
-                                                             T* __restrict__ U, int n, int ldq, int ldc, int r,
-                                                             int nqb, const double* __restrict__ D,
+                                                             T* __restrict__ U, std::size_t us, int n, int ldq,
+                                                             int ldc, int r, int nqb, const double* __restrict__ D,
                                                              double* __restrict__ rho_part) {
   using Acc = typename AccOf<T>::type;
   constexpr int BQ = 32 * WQ, BK = 16, BI = 32 * NIW, NT = 128 * WQ;
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(128 * WQ) void occupied_u_kernel(const T* __restric
       const int i = i0 + wi * 16 * NIW + y * 16 + (lane & 15);
       const int qb = q0 + wq * 32 + x * 16;
       if (i >= r || qb >= ldq) continue;
-      T* dst = U + (static_cast<std::size_t>(mu) * r + i) * ldq + qb;
+      T* dst = U + static_cast<std::size_t>(mu) * us + static_cast<std::size_t>(i) * ldq + qb;
 #pragma unroll
       for (int v = 0; v < 4; ++v) dst[acc_row<T>(lane, v)] = acc[x][y][v];
     }
@@ -351,7 +351,8 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
                                                                 const __bf16* __restrict__ Cth,
                                                                 const __bf16* __restrict__ Ctl,
                                                                 __bf16* __restrict__ Uh, __bf16* __restrict__ Ul,
-                                                                int n, int ldq, int ldct, int r, int nqb,
+                                                                std::size_t us, int n, int ldq, int ldct, int r,
+                                                                int nqb,
                                                                 const double* __restrict__ D,
                                                                 double* __restrict__ rho_part) {
   // 16 waves: WQ along Q x (16 / WQ) along i.  WQ = 4 gives 64 x 8*NIW wave
@@ -515,7 +516,7 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
       const int i = i0 + wi * 16 * YF + y * 16 + (lane & 15);
       const int qb = q0 + wq * 16 * XF + x * 16;
       if (i >= r || qb >= ldq) continue;
-      const std::size_t off = (static_cast<std::size_t>(mu) * r + i) * ldq + qb + 4 * (lane >> 4);
+      const std::size_t off = static_cast<std::size_t>(mu) * us + static_cast<std::size_t>(i) * ldq + qb + 4 * (lane >> 4);
 #pragma unroll
       for (int v = 0; v < 4; ++v) split_bf16(acc[x][y][v], Uh[off + v], Ul[off + v]);
     }
@@ -534,7 +535,7 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
 template <int NIW>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void occupied_u_bf16x3_direct_kernel(
     const float* __restrict__ B, const __bf16* __restrict__ Cth, const __bf16* __restrict__ Ctl,
-    __bf16* __restrict__ Uh, __bf16* __restrict__ Ul, int n, int ldq, int ldct, int r,
+    __bf16* __restrict__ Uh, __bf16* __restrict__ Ul, std::size_t us, int n, int ldq, int ldct, int r,
     const double* __restrict__ D, double* __restrict__ rho_part) {
   constexpr int NW = 8, NT = 64 * NW, BQ = 32 * NW, BK = 32, BI = 32 * NIW, PAD = 8;
   constexpr int XF = 2, YF = BI / 16;
@@ -673,7 +674,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void o
   for (int y = 0; y < YF; ++y) {
     const int i = i0 + y * 16 + (lane & 15);
     if (i >= r) continue;
-    const std::size_t off = (static_cast<std::size_t>(mu) * r + i) * ldq + qw + 8 * (lane >> 4);
+    const std::size_t off = static_cast<std::size_t>(mu) * us + static_cast<std::size_t>(i) * ldq + qw + 8 * (lane >> 4);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       split_bf16(acc[0][y][v], Uh[off + 2 * v], Ul[off + 2 * v]);
@@ -975,14 +976,14 @@ void launch_u(Precision prec, const void* B, const OccupiedOperands& ops, int ld
     const int nqb = (ldq + 32 * WQ - 1) / (32 * WQ);
     dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (32 * NIW));
     hipLaunchKernelGGL((occupied_u_kernel<double, NIW, WQ>), grid, dim3(128 * WQ), dyn, s,
-                       static_cast<const double*>(B), ops.c64, static_cast<double*>(U), ops.n, ldq, ops.ldc, ops.r,
-                       nqb, D, rho_part);
+                       static_cast<const double*>(B), ops.c64, static_cast<double*>(U), ops.ustride, ops.n, ldq,
+                       ops.ldc, ops.r, nqb, D, rho_part);
   } else if (prec == Precision::FP32) {
     constexpr int WQ = 8;
     const int nqb = (ldq + 32 * WQ - 1) / (32 * WQ);
     dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (32 * NIW));
     hipLaunchKernelGGL((occupied_u_kernel<float, NIW, WQ>), grid, dim3(128 * WQ), dyn, s, static_cast<const float*>(B),
-                       ops.c32, static_cast<float*>(U), ops.n, ldq, ops.ldc, ops.r, nqb, D, rho_part);
+                       ops.c32, static_cast<float*>(U), ops.ustride, ops.n, ldq, ops.ldc, ops.r, nqb, D, rho_part);
   } else if constexpr (NIW <= 3) {
     const int nqb = (ldq + 255) / 256;
     dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (32 * NIW));
@@ -991,8 +992,8 @@ void launch_u(Precision prec, const void* B, const OccupiedOperands& ops, int ld
                              (rho_part ? static_cast<std::size_t>(ops.n) * sizeof(float) : 0);
     hipLaunchKernelGGL((occupied_u_bf16x3_direct_kernel<NIW>), grid, dim3(512), ddyn, s, static_cast<const float*>(B),
                        static_cast<const __bf16*>(ops.ct_hi), static_cast<const __bf16*>(ops.ct_lo),
-                       static_cast<__bf16*>(U), static_cast<__bf16*>(U) + static_cast<std::size_t>(ops.n) * ops.r * ldq,
-                       ops.n, ldq, ops.ldct, ops.r, D, rho_part);
+                       static_cast<__bf16*>(U), static_cast<__bf16*>(U) + static_cast<std::size_t>(ops.n) * ops.ustride,
+                       ops.ustride, ops.n, ldq, ops.ldct, ops.r, D, rho_part);
   } else {
     const int nqb = (ldq + kUTileQ - 1) / kUTileQ;
     dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (32 * NIW));
@@ -1000,8 +1001,8 @@ void launch_u(Precision prec, const void* B, const OccupiedOperands& ops, int ld
     // 16-wave block allows once NIW >= 8.
     hipLaunchKernelGGL((occupied_u_bf16x3_kernel<NIW, 8, (NIW <= 4 ? 2 : 1)>), grid, dim3(1024), dyn, s, static_cast<const float*>(B),
                        static_cast<const __bf16*>(ops.ct_hi), static_cast<const __bf16*>(ops.ct_lo),
-                       static_cast<__bf16*>(U), static_cast<__bf16*>(U) + static_cast<std::size_t>(ops.n) * ops.r * ldq,
-                       ops.n, ldq, ops.ldct, ops.r, nqb, D, rho_part);
+                       static_cast<__bf16*>(U), static_cast<__bf16*>(U) + static_cast<std::size_t>(ops.n) * ops.ustride,
+                       ops.ustride, ops.n, ldq, ops.ldct, ops.r, nqb, D, rho_part);
   }
   VQC_LAUNCH("occupied_u_kernel");
 }
@@ -1145,9 +1146,16 @@ std::size_t exchange_partial_size(int n) {
   return static_cast<std::size_t>(syrk_split_cap(n) + 1) * 2 * static_cast<std::size_t>(n) * n;
 }
 
-void exchange_syrk(Precision prec, const void* U, int n, int r, int r_pos, int ldq, double* K, double* partial,
-                   hipStream_t s) {
+std::size_t u_row_stride(Precision prec, int r, int ldq) {
+  const std::size_t dense = static_cast<std::size_t>(r) * ldq;
+  if (prec == Precision::FP64) return dense;
+  return dense + (prec == Precision::FP32 ? 32 : 64);
+}
+
+void exchange_syrk(Precision prec, const void* U, std::size_t ustride, int n, int r, int r_pos, int ldq, double* K,
+                   double* partial, hipStream_t s) {
   const std::size_t kd = static_cast<std::size_t>(r) * ldq;
+  if (ustride < kd) throw std::invalid_argument("U row stride shorter than r x ldq");
   int slices = 0;
   auto run = [&](std::size_t kb, std::size_t ke, double sign) {
     if (ke <= kb) return;
@@ -1170,15 +1178,15 @@ void exchange_syrk(Precision prec, const void* U, int n, int r, int r_pos, int l
     const unsigned grid = static_cast<unsigned>(round_up(nwork, 8));
     if (prec == Precision::BF16X3) {
       const __bf16* wh = static_cast<const __bf16*>(U);
-      const __bf16* wl = wh + static_cast<std::size_t>(n) * kd;
-      hipLaunchKernelGGL(exchange_syrk_bf16x3_kernel, dim3(grid), dim3(512), 0, s, wh, wl, kd, n, tiles, ncb, nwork,
-                         kb, ke, chunk, sign, out32);
+      const __bf16* wl = wh + static_cast<std::size_t>(n) * ustride;
+      hipLaunchKernelGGL(exchange_syrk_bf16x3_kernel, dim3(grid), dim3(512), 0, s, wh, wl, ustride, n, tiles, ncb,
+                         nwork, kb, ke, chunk, sign, out32);
     } else if (prec == Precision::FP64) {
-      hipLaunchKernelGGL(exchange_syrk_kernel<double>, dim3(grid), dim3(512), 0, s, static_cast<const double*>(U), kd,
-                         n, tiles, ncb, nwork, kb, ke, chunk, sign, out64);
+      hipLaunchKernelGGL(exchange_syrk_kernel<double>, dim3(grid), dim3(512), 0, s, static_cast<const double*>(U),
+                         ustride, n, tiles, ncb, nwork, kb, ke, chunk, sign, out64);
     } else {
-      hipLaunchKernelGGL(exchange_syrk_kernel<float>, dim3(grid), dim3(512), 0, s, static_cast<const float*>(U), kd,
-                         n, tiles, ncb, nwork, kb, ke, chunk, sign, out32);
+      hipLaunchKernelGGL(exchange_syrk_kernel<float>, dim3(grid), dim3(512), 0, s, static_cast<const float*>(U),
+                         ustride, n, tiles, ncb, nwork, kb, ke, chunk, sign, out32);
     }
     VQC_LAUNCH("exchange_syrk_kernel");
     slices += splits;

@@ -39,23 +39,32 @@ void coulomb_matrix(Precision prec, const void* B, const double* rho, std::size_
 // C64: n x ldc FP64 factor.  Prepares the precision-specific operand copies.
 struct OccupiedOperands {
   int n = 0, r = 0, ldc = 0, ni = 1, ldct = 0;
+  std::size_t ustride = 0;    // elements between consecutive mu rows of U (u_row_stride)
   const double* c64 = nullptr;
   float* c32 = nullptr;       // n x ldc
   void* ct_hi = nullptr;      // ldc x ldct bf16 (transposed), hi part
   void* ct_lo = nullptr;      // lo part
 };
 void prepare_occupied(Precision prec, OccupiedOperands& ops, hipStream_t s);
+// Elements between the rows U[mu] and U[mu + 1] (each r x ldq).  The SYRK
+// reads the same k window of every row at once; with a row stride of a large
+// power of two times an odd number those reads fall on a few cache sets and
+// memory channels.  ldq is a multiple of 128, so r ldq is an even number of
+// 128-byte lines in FP32 and BF16; the reduced-precision layouts add one line
+// (32 FP32 or 64 BF16 elements) so that consecutive rows start an odd number
+// of lines apart.  FP64 keeps r ldq (the gradient reads FP64 U densely).
+std::size_t u_row_stride(Precision prec, int r, int ldq);
 // U[mu][i][Q] (r x ldq per mu): FP64 (fp64), FP32 (fp32), or hi/lo bfloat16
 // planes (bf16x3).  With rho_part non-null the same pass writes
 // rho_part[mu][Q] = sum_nu D[mu][nu] B[(mu,nu)][Q] (n x ldq, FP64).
 void occupied_u(Precision prec, const void* B, const OccupiedOperands& ops, int ldq, void* U, const double* D,
                 double* rho_part, hipStream_t s);
 // K (n x n, FP64, both triangles) = sum_k sign_k W[mu][k] W[lam][k] where
-// columns i < r_pos carry +1 and the rest -1.  `partial` needs
-// exchange_partial_size(n) doubles.
+// W[mu] = U[mu] (r ldq elements, rows ustride apart) and columns i < r_pos
+// carry +1, the rest -1.  `partial` needs exchange_partial_size(n) doubles.
 std::size_t exchange_partial_size(int n);
-void exchange_syrk(Precision prec, const void* U, int n, int r, int r_pos, int ldq, double* K, double* partial,
-                   hipStream_t s);
+void exchange_syrk(Precision prec, const void* U, std::size_t ustride, int n, int r, int r_pos, int ldq, double* K,
+                   double* partial, hipStream_t s);
 
 // ---- conversions ---------------------------------------------------------
 void convert_f64_to_f32(const double* in, float* out, std::size_t count, hipStream_t s);

@@ -310,11 +310,12 @@ void DfPlan::build_jk(Precision p, const double* D, int r, int r_pos) {
   const bool fused_rho = opts_.exchange && opts_.coulomb && r > 0;
   if (opts_.exchange) {
     ops_.r = r;
+    ops_.ustride = u_row_stride(p, std::max(r, 1), ldq_);
     if (r > 0) {
       prepare_occupied(p, ops_, stream_);
       occupied_u(p, B, ops_, ldq_, U_.data(), D, fused_rho ? rho_partial_.data() : nullptr, stream_);
     }
-    exchange_syrk(p, U_.data(), n_, r, r_pos, ldq_, K_.data(), syrk_partial_.data(), stream_);
+    exchange_syrk(p, U_.data(), ops_.ustride, n_, r, r_pos, ldq_, K_.data(), syrk_partial_.data(), stream_);
   }
   if (opts_.coulomb) {
     if (fused_rho) {
@@ -461,10 +462,11 @@ std::array<double, 5> DfPlan::benchmark_occupied(const std::vector<double>& occu
   for (unsigned it = 0; it < repeat; ++it) {
     VQC_HIP(hipEventRecord(ev[it * 4 + 0].e, stream_));
     ops_.r = nocc;
+    ops_.ustride = u_row_stride(p, nocc, ldq_);
     prepare_occupied(p, ops_, stream_);
     occupied_u(p, B, ops_, ldq_, U_.data(), D_.data(), rho_partial_.data(), stream_);
     VQC_HIP(hipEventRecord(ev[it * 4 + 1].e, stream_));
-    exchange_syrk(p, U_.data(), n_, nocc, nocc, ldq_, K_.data(), syrk_partial_.data(), stream_);
+    exchange_syrk(p, U_.data(), ops_.ustride, n_, nocc, nocc, ldq_, K_.data(), syrk_partial_.data(), stream_);
     VQC_HIP(hipEventRecord(ev[it * 4 + 2].e, stream_));
     rho_from_partials(rho_partial_.data(), n_, ldq_, rho_.data(), stream_);
     coulomb_matrix(p, B, rho_.data(), npair_, ldq_, n_, J_.data(), stream_);

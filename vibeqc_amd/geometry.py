@@ -63,6 +63,16 @@ def water_cluster(n: int, box_angstrom: float, seed: int, displacement_angstrom:
     return tuple(Atom(zz, tuple(map(float, p))) for zz, p in zip(z, xyz))
 
 
+def ragged_water_batch(count: int = 512, seed: int = 2):
+    """The ragged benchmark batch: (H2O)n with n ~ U{1..16} drawn from `seed`,
+    member k built by water_cluster(n, 3 n^(1/3) A, seed=1000 + k).
+    Returns (systems, sizes)."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(1, 17, size=count)
+    systems = [water_cluster(int(n), 3.0 * float(n) ** (1.0 / 3.0), seed=1000 + k) for k, n in enumerate(sizes)]
+    return systems, sizes
+
+
 def alkane(n_carbon: int):
     """All-trans CnH(2n+2) zig-zag in the xy plane (Bohr)."""
     cc, ch = 1.54, 1.09
