@@ -200,6 +200,7 @@ class FockDiagnosticC(ct.Structure):
 
 METHOD_RHF = 1
 METHOD_UHF = 2
+DENSITY_FITTING_NONE = 0
 DENSITY_FITTING_CUDA = 2
 PRECISION_MODE = {"fp64": 0, "auto": 1}
 BATCH_WARM_STARTS = 1
@@ -281,6 +282,23 @@ class HfWarmState(ct.Structure):
     ]
 
 
+class ResultDescriptorC(ct.Structure):
+    """generativeqc_result_descriptor (generativeqc.h:1067-1081)."""
+
+    _fields_ = [
+        ("struct_size", ct.c_uint32),
+        ("abi_version", ct.c_uint32),
+        ("energy", ct.c_double),
+        ("forces", _DP),
+        ("force_count", ct.c_uint32),
+        ("iterations", ct.c_uint32),
+        ("energy_change", ct.c_double),
+        ("density_rms", ct.c_double),
+        ("converged", ct.c_int32),
+        ("executed_backend", ct.c_int32),
+    ]
+
+
 def descriptor(kind, **values):
     """A versioned descriptor with struct_size / abi_version filled in."""
     return kind(struct_size=ct.sizeof(kind), abi_version=ABI_VERSION, **values)
@@ -348,6 +366,14 @@ EXPORTS = {
     "generativeqc_batch_get_hf_warm_state": (ct.c_int32, [ct.c_void_p, ct.c_uint32, ct.POINTER(HfWarmState)]),
     "generativeqc_batch_restore_hf_warm_states": (ct.c_int32, [ct.c_void_p, ct.POINTER(HfWarmState), ct.c_uint32]),
     "generativeqc_batch_clear_warm_starts": (ct.c_int32, [ct.c_void_p]),
+    "generativeqc_calculation_prepare": (
+        ct.c_int32,
+        [ct.c_void_p, ct.c_void_p, ct.POINTER(MethodDescriptorC), ct.POINTER(ct.c_void_p)],
+    ),
+    "generativeqc_calculation_destroy": (None, [ct.c_void_p]),
+    "generativeqc_calculation_execute": (ct.c_int32, [ct.c_void_p, ct.POINTER(ResultDescriptorC)]),
+    "vibeqc_calculation_set_precision": (ct.c_int32, [ct.c_void_p, ct.c_int32]),
+    "vibeqc_calculation_last_state": (ct.c_int32, [ct.c_void_p, _DP, ct.c_uint64, ct.POINTER(ct.c_uint64)]),
 }
 
 

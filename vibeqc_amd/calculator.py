@@ -115,11 +115,14 @@ class Calculator:
         )
         return orbital, aux
 
-    def _method_descriptor(self):
-        """generativeqc_method_descriptor of this calculator (generativeqc.h:723-769)."""
+    def _method_descriptor(self, auxiliary=None, *, single=False):
+        """generativeqc_method_descriptor of this calculator (generativeqc.h:723-769).
+        ``auxiliary`` is an optional native system handle carrying the
+        auxiliary shells; a single calculation (``single``) also accepts fp32,
+        selected after preparation."""
         from . import _native
 
-        if self.precision not in ("fp64", "mixed", "bf16x3"):
+        if self.precision not in ("fp64", "mixed", "bf16x3") and not (single and self.precision == "fp32"):
             raise NotImplementedError("batches run in fp64 or mixed (bf16x3 + FP64 refinement) precision")
         return _native.descriptor(
             _native.MethodDescriptorC,
@@ -130,10 +133,10 @@ class Calculator:
             density_tolerance=self.density_tolerance,
             screening_tolerance=self.screening_tolerance,
             density_fitting_mode=_native.DENSITY_FITTING_CUDA,
-            density_fitting_auxiliary_basis=None,
+            density_fitting_auxiliary_basis=auxiliary,
             density_fitting_relative_threshold=self.metric_threshold,
             density_fitting_memory_budget_bytes=self.budget,
-            precision_mode=_native.PRECISION_MODE["fp64" if self.precision == "fp64" else "auto"],
+            precision_mode=_native.PRECISION_MODE["auto" if self.precision in ("mixed", "bf16x3") else "fp64"],
         )
 
     def prepare_batch(self, systems, *, charges=None, multiplicities=None, warm_start: bool = True):
@@ -177,27 +180,56 @@ class Calculator:
         unknown = requested - {"energy", "forces"}
         if unknown:
             raise ValueError(f"method {self.method!r} does not support properties: {', '.join(sorted(map(repr, unknown)))}")
-        with self.prepare(atoms, charge=charge, multiplicity=multiplicity) as plan:
-            r = plan.solve(
-                compute_forces="forces" in requested,
-                max_iterations=self.max_iterations,
-                diis_history=self.diis_history,
-                energy_tolerance=self.energy_tolerance,
-                density_tolerance=self.density_tolerance,
+        return self._calculation(_normalise_atoms(atoms), charge, multiplicity, "forces" in requested)
+
+    def _calculation(self, atoms, charge, multiplicity, want_forces) -> Result:
+        """One prepared native calculation: generativeqc_calculation_prepare /
+        _execute / _destroy, bound as the reference's Calculator binds them
+        (calculator.py:2291-2346)."""
+        import ctypes as ct
+
+        from . import _native
+        from .fock import NativeContext
+
+        orbital, aux = self._bases(atoms, charge, multiplicity)
+        ctx = NativeContext(self.device_id)
+        lib = ctx.lib
+        handles, calc = [], ct.c_void_p()
+        try:
+            handles = [ctx.system(orbital), ctx.system(aux)]
+            desc = self._method_descriptor(handles[1], single=True)
+            status = lib.generativeqc_calculation_prepare(ctx.handle, handles[0], ct.byref(desc), ct.byref(calc))
+            _native.check(status, lib.generativeqc_context_get_last_detail(ctx.handle))
+            if self.precision == "fp32":
+                _native.check(lib.vibeqc_calculation_set_precision(calc, _native.PRECISION["fp32"]),
+                              lib.generativeqc_context_get_last_detail(ctx.handle))
+            forces = np.empty((len(atoms), 3)) if want_forces else None
+            result = _native.descriptor(
+                _native.ResultDescriptorC,
+                forces=_native.pointer(forces),
+                force_count=0 if forces is None else forces.size,
+                executed_backend=_native.BACKEND_HIP,
             )
-            shape = plan.shape()
-        return Result(
-            r.energy,
-            r.forces,
-            r.iterations,
-            True,
-            r.energy_change,
-            r.density_rms,
-            r.fock_builds,
-            r.density,
-            shape["nbf"],
-            shape["naux"],
-        )
+            status = lib.generativeqc_calculation_execute(calc, ct.byref(result))
+            _native.check(status, lib.generativeqc_context_get_last_detail(ctx.handle))
+            counts = (ct.c_uint64 * 4)()
+            _native.check(lib.vibeqc_calculation_last_state(calc, None, 0, counts), "calculation state query failed")
+            density = np.empty(counts[0])
+            _native.check(lib.vibeqc_calculation_last_state(calc, _native.pointer(density), counts[0], counts),
+                          "calculation state copy failed")
+            n = int(counts[2])
+            density = density.reshape((2, n, n) if self.method == "uhf" else (n, n))
+            density.setflags(write=False)
+            if forces is not None:
+                forces.setflags(write=False)
+            return Result(result.energy, forces, result.iterations, bool(result.converged), result.energy_change,
+                          result.density_rms, int(counts[1]), density, n, int(counts[3]))
+        finally:
+            if calc:
+                lib.generativeqc_calculation_destroy(calc)
+            for h in handles:
+                lib.generativeqc_system_destroy(h)
+            ctx.close()
 
 
 __all__ = ["Calculator", "Result", "atoms_from"]

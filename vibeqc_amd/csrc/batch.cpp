@@ -10,6 +10,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <limits>
 #include <map>
 #include <mutex>
@@ -20,23 +21,25 @@
 namespace vqc {
 namespace {
 
-// Keep freed stream-ordered memory in the device pool for the next plan
-// rather than releasing it back to the system at every synchronisation.
-void retain_pool(int device) {
-  hipMemPool_t pool = nullptr;
-  if (hipDeviceGetDefaultMemPool(&pool, device) != hipSuccess || !pool) return;
-  uint64_t threshold = std::numeric_limits<uint64_t>::max();
-  (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
-}
-
-// Routes this thread's device allocations to a worker's stream for the
-// lifetime of the object.
+// Routes this thread's device allocations to a worker's stream and block
+// cache for the lifetime of the object; nested scopes restore the enclosing one.
 class AllocationScope {
  public:
-  explicit AllocationScope(hipStream_t s) { allocation_stream() = s; }
-  ~AllocationScope() { allocation_stream() = nullptr; }
+  AllocationScope(hipStream_t s, BlockCache* cache)
+      : previous_(allocation_stream()), previous_cache_(allocation_cache()) {
+    allocation_stream() = s;
+    allocation_cache() = cache;
+  }
+  ~AllocationScope() {
+    allocation_stream() = previous_;
+    allocation_cache() = previous_cache_;
+  }
   AllocationScope(const AllocationScope&) = delete;
   AllocationScope& operator=(const AllocationScope&) = delete;
+
+ private:
+  hipStream_t previous_;
+  BlockCache* previous_cache_;
 };
 
 void move_atoms(System& sys, const std::vector<double>& xyz) {
@@ -124,7 +127,6 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
     throw std::invalid_argument("batch coordinate count mismatch");
   if (forces.size() != count) throw std::invalid_argument("batch force request count mismatch");
   VQC_HIP(hipSetDevice(device_));
-  retain_pool(device_);
   std::vector<BatchItemOutcome> out(count);
   // Worker count: the option, else VIBEQC_BATCH_WORKERS, else 8.
   int requested = opts_.workers;
@@ -135,6 +137,8 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
   // after all plans that borrow them.
   std::vector<hipStream_t> streams(workers, nullptr);
   std::vector<rocblas_handle> handles(workers, nullptr);
+  // Device memory of each worker's plans (recycled within the worker only).
+  std::vector<BlockCache> caches(workers);
   auto release_resources = [&]() {
     for (int w = 0; w < workers; ++w) {
       if (streams[w]) (void)hipStreamSynchronize(streams[w]);
@@ -158,8 +162,11 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
   // are set up one after another (integral generation, metric factorisation,
   // B), then their SCFs run concurrently, one worker per stream, and forces
   // are evaluated one system at a time.  Setup and derivative-integral
-  // kernels therefore never share the device with another system's work: when
-  // they did, results were perturbed at the 1e-8..1e-4 Eh level; with the
+  // kernels therefore never share the device with another system's work.
+  // When setups ran on several hardware queues at once, identical device
+  // inputs (checksummed) produced different metric, three-centre and
+  // Cholesky results (energies off by up to 0.5 Eh); the same parallel setups
+  // funnelled through one hardware queue were bitwise reproducible.  With the
   // phases separated, repeated executions are bitwise reproducible.
   const std::size_t wave = static_cast<std::size_t>(workers) * 4;
   std::vector<std::unique_ptr<DfPlan>> plans(count);
@@ -168,7 +175,7 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
     const int w = static_cast<int>(k % workers);
     BatchItemOutcome& r = out[i];
     r.bucket = bucket_[i];
-    AllocationScope scope(streams[w]);
+    AllocationScope scope(streams[w], &caches[w]);
     try {
       System sys = systems_[i];
       System aux = aux_[i];
@@ -212,6 +219,34 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
       r.error = std::current_exception();
     }
   };
+  // Forces and the retained seed of a converged item (on worker k % workers).
+  auto finish_item = [&](std::size_t k) {
+    const std::size_t i = order_[k];
+    BatchItemOutcome& r = out[i];
+    if (!plans[k] || r.error || !r.scf.converged) return;
+    try {
+      if (forces[i]) {
+        AllocationScope scope(streams[k % workers], &caches[k % workers]);
+        r.scf.forces = plans[k]->converged_forces();
+      }
+      if (opts_.warm_starts && warm_updates_) {
+        WarmState& ws = warm_[i];
+        ws.density = r.scf.density;
+        if (!coordinates.empty() && coordinates[i]) {
+          ws.coordinates = *coordinates[i];
+        } else {
+          ws.coordinates.clear();
+          for (const auto& a : systems_[i].atoms) ws.coordinates.insert(ws.coordinates.end(), a.xyz, a.xyz + 3);
+        }
+        ws.energy = r.scf.energy;
+        ws.energy_change = r.scf.energy_change;
+        ws.density_rms = r.scf.density_rms;
+        ws.iterations = static_cast<int>(r.scf.iterations);
+      }
+    } catch (...) {
+      r.error = std::current_exception();
+    }
+  };
   // VIBEQC_TRACE set: one JSON line per wave with its phase times.
   const bool trace = std::getenv("VIBEQC_TRACE") != nullptr;
   using clock = std::chrono::steady_clock;
@@ -225,7 +260,7 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
       const auto t_scf = clock::now();
       auto run = [&](int w) {
         if (hipSetDevice(device_) != hipSuccess) return;
-        AllocationScope scope(streams[w]);
+        AllocationScope scope(streams[w], &caches[w]);
         for (std::size_t k = w0 + static_cast<std::size_t>(w); k < w1; k += workers) solve_item(k);
         (void)hipStreamSynchronize(streams[w]);
       };
@@ -235,33 +270,7 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
       for (auto& t : pool) t.join();
       const double scf_ms = ms_since(t_scf);
       const auto t_forces = clock::now();
-      for (std::size_t k = w0; k < w1; ++k) {
-        const std::size_t i = order_[k];
-        BatchItemOutcome& r = out[i];
-        if (!plans[k] || r.error || !r.scf.converged) continue;
-        try {
-          if (forces[i]) {
-            AllocationScope scope(streams[k % workers]);
-            r.scf.forces = plans[k]->converged_forces();
-          }
-          if (opts_.warm_starts && warm_updates_) {
-            WarmState& ws = warm_[i];
-            ws.density = r.scf.density;
-            if (!coordinates.empty() && coordinates[i]) {
-              ws.coordinates = *coordinates[i];
-            } else {
-              ws.coordinates.clear();
-              for (const auto& a : systems_[i].atoms) ws.coordinates.insert(ws.coordinates.end(), a.xyz, a.xyz + 3);
-            }
-            ws.energy = r.scf.energy;
-            ws.energy_change = r.scf.energy_change;
-            ws.density_rms = r.scf.density_rms;
-            ws.iterations = static_cast<int>(r.scf.iterations);
-          }
-        } catch (...) {
-          r.error = std::current_exception();
-        }
-      }
+      for (std::size_t k = w0; k < w1; ++k) finish_item(k);
       const double forces_ms = ms_since(t_forces);
       for (std::size_t k = w0; k < w1; ++k) plans[k].reset();
       if (trace)

@@ -4,9 +4,10 @@
 //
 // Systems are grouped into shape buckets (nbf, naux, nocc), as the reference
 // groups its fleets (fleet.cpp:26-36), and executed largest first in waves of
-// 4 x workers systems.  Each host worker owns one HIP stream and one rocBLAS
-// handle and allocates stream-ordered, so the SCFs of a wave run concurrently
-// on the GPU instead of one after another.  A plan lives only for its wave,
+// 4 x workers systems.  Each host worker owns one HIP stream, one rocBLAS
+// handle and one device block cache (blocks recycled only among the worker's
+// own buffers, so no block moves between streams), and the SCFs of a wave
+// run concurrently on the GPU instead of one after another.  A plan lives only for its wave,
 // which bounds device memory by the largest wave rather than the whole batch.
 #pragma once
 

@@ -19,6 +19,7 @@
 #include "batch.hpp"
 #include "common.hpp"
 #include "generativeqc/batch.h"
+#include "generativeqc/calculation.h"
 #include "generativeqc/fock.h"
 #include "generativeqc/generativeqc.h"
 #include "plan.hpp"
@@ -38,6 +39,17 @@ struct generativeqc_batch {
   std::vector<std::size_t> natoms;
   std::vector<uint64_t> fock_builds;
   std::string detail;
+};
+
+struct generativeqc_calculation {
+  generativeqc_context* context = nullptr;
+  std::unique_ptr<vqc::DfPlan> plan;
+  vqc::ScfOptions scf;
+  bool unrestricted = false;
+  std::size_t natom = 0;
+  uint64_t budget = 0;
+  std::vector<double> density;  // last successful execution
+  uint64_t fock_builds = 0;
 };
 
 struct generativeqc_fock_plan {
@@ -127,10 +139,10 @@ void copy_text(char* dst, std::size_t cap, const char* src) {
 // RHF/DF options of a method descriptor (reference generativeqc.h:723-769).
 vqc::BatchOptions batch_options(const generativeqc_method_descriptor& d, generativeqc_batch_flags flags) {
   if (d.method != GENERATIVEQC_METHOD_RHF && d.method != GENERATIVEQC_METHOD_UHF)
-    throw vqc::NotImplemented("batches implement the RHF and UHF methods only");
+    throw vqc::NotImplemented("the RHF and UHF methods are implemented");
   if (d.density_fitting_mode != GENERATIVEQC_DENSITY_FITTING_CUDA &&
       d.density_fitting_mode != GENERATIVEQC_DENSITY_FITTING_AUTO)
-    throw vqc::NotImplemented("batches run density-fitted J/K on the HIP device only");
+    throw vqc::NotImplemented("density-fitted J/K on the HIP device only (exact J/K and the CPU backend are not implemented)");
   if (d.ks_options || d.initial_guess) throw vqc::NotImplemented("KS and initial-guess options are not implemented");
   require(d.precision_mode == GENERATIVEQC_PRECISION_FP64 || d.precision_mode == GENERATIVEQC_PRECISION_AUTO,
           "unknown precision mode");
@@ -751,12 +763,16 @@ generativeqc_status generativeqc_batch_execute(generativeqc_batch* batch, const 
       r.warm_start_fallback = o.warm_start_fallback;
       batch->fock_builds[i] = o.scf.fock_builds;
       if (o.error) {
-        std::string ignored;
+        std::string what;
         try {
           std::rethrow_exception(o.error);
         } catch (...) {
-          r.status = map_exception(&ignored);
+          r.status = map_exception(&what);
         }
+        // VIBEQC_TRACE set: the failed item's detail on stderr.
+        if (std::getenv("VIBEQC_TRACE"))
+          std::fprintf(stderr, "{\"trace\": \"batch_item_error\", \"index\": %zu, \"status\": %d, \"detail\": \"%s\"}\n", i,
+                       static_cast<int>(r.status), what.c_str());
         r.converged = 0;
         continue;
       }
@@ -776,6 +792,104 @@ generativeqc_status generativeqc_batch_execute(generativeqc_batch* batch, const 
   } catch (...) {
     return map_exception(&batch->detail);
   }
+}
+
+}  // extern "C"
+
+extern "C" {
+
+generativeqc_status generativeqc_calculation_prepare(generativeqc_context* context, const generativeqc_system* system,
+                                                     const generativeqc_method_descriptor* descriptor,
+                                                     generativeqc_calculation** calculation) {
+  if (!context || !system || !descriptor || !calculation) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  *calculation = nullptr;
+  if (!valid_descriptor(descriptor)) return GENERATIVEQC_STATUS_ABI_MISMATCH;
+  std::lock_guard<std::recursive_mutex> lock(context->mutex);
+  try {
+    // The batch's descriptor rules (methods, fitting modes, defaults).
+    const vqc::BatchOptions opts = batch_options(*descriptor, 0);
+    auto out = std::make_unique<generativeqc_calculation>();
+    out->context = context;
+    out->scf = opts.scf;
+    out->unrestricted = opts.unrestricted;
+    out->natom = system->data.atoms.size();
+    out->budget = descriptor->density_fitting_memory_budget_bytes;
+    const vqc::System aux = descriptor->density_fitting_auxiliary_basis
+                                ? auxiliary_from_template(descriptor->density_fitting_auxiliary_basis->data, system->data)
+                                : system->data;
+    out->plan = std::make_unique<vqc::DfPlan>(context->device, system->data, aux, opts.plan);
+    out->plan->set_precision(opts.precision);
+    *calculation = out.release();
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(&context->detail);
+  }
+}
+
+void generativeqc_calculation_destroy(generativeqc_calculation* calculation) { delete calculation; }
+
+generativeqc_status generativeqc_calculation_execute(generativeqc_calculation* calculation,
+                                                     generativeqc_result_descriptor* result) {
+  if (!calculation) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  generativeqc_context* context = calculation->context;
+  std::lock_guard<std::recursive_mutex> lock(context->mutex);
+  calculation->density.clear();
+  calculation->fock_builds = 0;
+  if (!result) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  if (!valid_descriptor(result)) return GENERATIVEQC_STATUS_ABI_MISMATCH;
+  const bool omit_forces = result->forces == nullptr && result->force_count == 0;
+  if (!result->forces && !omit_forces) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  if (!omit_forces && result->force_count < calculation->natom * 3) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  try {
+    vqc::DfPlan& plan = *calculation->plan;
+    const vqc::ScfOutcome o = calculation->unrestricted ? plan.solve_unrestricted(calculation->scf, nullptr, !omit_forces)
+                                                        : plan.solve(calculation->scf, nullptr, !omit_forces);
+    result->energy = o.energy;
+    result->iterations = o.iterations;
+    result->energy_change = o.energy_change;
+    result->density_rms = o.density_rms;
+    result->converged = o.converged ? 1 : 0;
+    result->executed_backend = GENERATIVEQC_BACKEND_CUDA;
+    if (!o.converged) {
+      context->detail = "SCF did not converge";
+      return GENERATIVEQC_STATUS_NOT_CONVERGED;
+    }
+    if (!omit_forces) {
+      if (o.forces.size() > result->force_count) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+      std::copy(o.forces.begin(), o.forces.end(), result->forces);
+    }
+    calculation->density = o.density;
+    calculation->fock_builds = o.fock_builds;
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(&context->detail);
+  }
+}
+
+generativeqc_status vibeqc_calculation_set_precision(generativeqc_calculation* calculation, int32_t precision) {
+  if (!calculation || precision < 0 || precision > 2) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  std::lock_guard<std::recursive_mutex> lock(calculation->context->mutex);
+  try {
+    calculation->plan->set_precision(static_cast<vqc::Precision>(precision));
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(&calculation->context->detail);
+  }
+}
+
+generativeqc_status vibeqc_calculation_last_state(const generativeqc_calculation* calculation, double* density,
+                                                  uint64_t density_count, uint64_t* counts4) {
+  if (!calculation || !counts4) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  const auto& d = calculation->density;
+  if (density) {
+    if (d.empty() || density_count != d.size()) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+    std::copy(d.begin(), d.end(), density);
+  }
+  counts4[0] = d.size();
+  counts4[1] = calculation->fock_builds;
+  counts4[2] = static_cast<uint64_t>(calculation->plan->nbf());
+  counts4[3] = static_cast<uint64_t>(calculation->plan->naux());
+  return GENERATIVEQC_STATUS_SUCCESS;
 }
 
 }  // extern "C"

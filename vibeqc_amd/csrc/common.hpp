@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -130,6 +131,55 @@ inline hipStream_t& allocation_stream() {
   return stream;
 }
 
+// Device blocks owned by one batch worker, recycled only among that worker's
+// own allocations.  Every buffer a worker allocates is used by work on the
+// worker's stream only, so a block released to the cache and handed out again
+// is touched by its next user's work strictly after its previous user's work
+// (stream order): no block ever moves between streams while work may still
+// use it.  Blocks are returned to the device with hipFree when the cache is
+// destroyed (after the worker's stream has drained).
+class BlockCache {
+ public:
+  BlockCache() = default;
+  BlockCache(const BlockCache&) = delete;
+  BlockCache& operator=(const BlockCache&) = delete;
+  ~BlockCache() {
+    for (auto& b : free_) (void)hipFree(b.second);
+  }
+  // A block of at least `bytes` (sizes rounded up to 4 KiB; a cached block is
+  // reused when it is at most twice the request).
+  void* take(std::size_t bytes) {
+    bytes = (std::max<std::size_t>(bytes, 1) + 4095) & ~static_cast<std::size_t>(4095);
+    auto it = free_.lower_bound(bytes);
+    if (it != free_.end() && it->first <= 2 * bytes) {
+      void* p = it->second;
+      sizes_[p] = it->first;
+      free_.erase(it);
+      return p;
+    }
+    void* p = nullptr;
+    hip_check(hipMalloc(&p, bytes), "hipMalloc");
+    sizes_[p] = bytes;
+    return p;
+  }
+  void give(void* p) {
+    auto it = sizes_.find(p);
+    if (it == sizes_.end()) return;
+    free_.emplace(it->second, p);
+    sizes_.erase(it);
+  }
+
+ private:
+  std::multimap<std::size_t, void*> free_;
+  std::map<void*, std::size_t> sizes_;
+};
+
+// Cache used by DeviceArrays created on this host thread (null: none).
+inline BlockCache*& allocation_cache() {
+  static thread_local BlockCache* cache = nullptr;
+  return cache;
+}
+
 // Owned device allocation; moves, never copies.
 template <class T>
 class DeviceArray {
@@ -139,13 +189,17 @@ class DeviceArray {
   DeviceArray(const DeviceArray&) = delete;
   DeviceArray& operator=(const DeviceArray&) = delete;
   DeviceArray(DeviceArray&& o) noexcept
-      : p_(std::exchange(o.p_, nullptr)), n_(std::exchange(o.n_, 0)), s_(std::exchange(o.s_, nullptr)) {}
+      : p_(std::exchange(o.p_, nullptr)),
+        n_(std::exchange(o.n_, 0)),
+        s_(std::exchange(o.s_, nullptr)),
+        cache_(std::exchange(o.cache_, nullptr)) {}
   DeviceArray& operator=(DeviceArray&& o) noexcept {
     if (this != &o) {
       release();
       p_ = std::exchange(o.p_, nullptr);
       n_ = std::exchange(o.n_, 0);
       s_ = std::exchange(o.s_, nullptr);
+      cache_ = std::exchange(o.cache_, nullptr);
     }
     return *this;
   }
@@ -156,8 +210,10 @@ class DeviceArray {
     if (n) {
       void* q = nullptr;
       s_ = allocation_stream();
+      cache_ = allocation_cache();
       const std::size_t bytes = n * sizeof(T) + (guard_allocations() ? kGuardBytes : 0);
-      if (s_) hip_check(hipMallocAsync(&q, bytes, s_), "hipMallocAsync");
+      if (cache_) q = cache_->take(bytes);
+      else if (s_) hip_check(hipMallocAsync(&q, bytes, s_), "hipMallocAsync");
       else hip_check(hipMalloc(&q, bytes), "hipMalloc");
       if (guard_allocations()) {
         hip_check(hipMemsetAsync(static_cast<char*>(q) + n * sizeof(T), 0xA5, kGuardBytes, s_), "guard");
@@ -186,12 +242,14 @@ class DeviceArray {
       }
     }
     if (p_) {
-      if (s_) (void)hipFreeAsync(p_, s_);
+      if (cache_) cache_->give(p_);
+      else if (s_) (void)hipFreeAsync(p_, s_);
       else (void)hipFree(p_);
     }
     p_ = nullptr;
     n_ = 0;
     s_ = nullptr;
+    cache_ = nullptr;
   }
   T* data() const { return p_; }
   std::size_t size() const { return n_; }
@@ -215,6 +273,7 @@ class DeviceArray {
   T* p_ = nullptr;
   std::size_t n_ = 0;
   hipStream_t s_ = nullptr;  // stream of a stream-ordered allocation
+  BlockCache* cache_ = nullptr;  // worker cache the block came from
 };
 
 inline std::size_t pair_count(std::size_t n) { return n * (n + 1) / 2; }

@@ -4,6 +4,8 @@
 #pragma once
 
 #include <cmath>
+#include <type_traits>
+#include <utility>
 
 #include "integrals.hpp"
 
@@ -16,6 +18,36 @@ __host__ __device__ constexpr int tuv(int t, int u, int v) {
   return s * (s + 1) * (s + 2) / 6 + t * (s + 1) - t * (t - 1) / 2 + u;
 }
 __host__ __device__ constexpr int nc(int l) { return (l + 1) * (l + 2) / 2; }
+// Inverse of tuv(): the degree s, t and u of the e-th packed entry.
+__host__ __device__ constexpr int tuv_degree(int e) {
+  int s = 0;
+  while (ntuv(s) <= e) ++s;
+  return s;
+}
+__host__ __device__ constexpr int tuv_t(int e) {
+  const int s = tuv_degree(e);
+  int r = e - (s == 0 ? 0 : ntuv(s - 1)), t = 0;
+  while (r >= s - t + 1) {
+    r -= s - t + 1;
+    ++t;
+  }
+  return t;
+}
+__host__ __device__ constexpr int tuv_u(int e) {
+  const int s = tuv_degree(e);
+  int r = e - (s == 0 ? 0 : ntuv(s - 1)), t = 0;
+  while (r >= s - t + 1) {
+    r -= s - t + 1;
+    ++t;
+  }
+  return r;
+}
+// Position of (t, u, v) among the degree-(t+u+v) entries (tuv() minus the
+// entries of lower degree).
+__host__ __device__ constexpr int tuv_pos(int t, int u, int v) {
+  const int s = t + u + v;
+  return t * (s + 1) - t * (t - 1) / 2 + u;
+}
 __host__ __device__ constexpr int cart_lx(int l, int c) {
   int lx = l;
   while (c > l - lx) {
@@ -62,6 +94,19 @@ __host__ __device__ constexpr double binom(int n, int k) {
   return r;
 }
 
+// Compile-time loop: f(std::integral_constant<int, I>) for I = 0..N-1, so
+// every register-array index derived from I is a constant expression (a
+// run-time index would put the array in per-lane scratch memory).
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (N > 0) static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+#define VQC_CI(x) decltype(x)::value
+
 template <int M>
 __device__ __forceinline__ void boys_eval(double T, const double* __restrict__ table, double (&F)[M + 1]) {
   const double e = exp(-T);
@@ -88,7 +133,10 @@ __device__ __forceinline__ void boys_eval(double T, const double* __restrict__ t
   }
 }
 
-// R_{tuv} = R^{(0)}_{tuv} for t+u+v <= L, in place from the highest degree down.
+// R_{tuv} = R^{(0)}_{tuv} for t+u+v <= L, in place from the highest degree down:
+// at level n the degree-s entries come from the level n+1 entries of degrees
+// s-1 and s-2, which are still in place because lower degrees are updated
+// later (R_{t+1,u,v} = X R_{t,u,v} + t R_{t-1,u,v}, likewise in u and v).
 template <int L>
 __device__ __forceinline__ void hermite_r(double alpha, double X, double Y, double Z, const double (&F)[L + 1],
                                           double (&R)[ntuv(L)]) {
@@ -97,32 +145,31 @@ __device__ __forceinline__ void hermite_r(double alpha, double X, double Y, doub
 #pragma unroll
   for (int n = 1; n <= L; ++n) pw[n] = pw[n - 1] * (-2.0 * alpha);
   R[0] = pw[L] * F[L];
-#pragma unroll
-  for (int n = L - 1; n >= 0; --n) {
-#pragma unroll
-    for (int s = L - n; s >= 1; --s) {
-#pragma unroll
-      for (int t = 0; t <= s; ++t) {
-#pragma unroll
-        for (int u = 0; u <= s - t; ++u) {
-          const int v = s - t - u;
+  static_for<L>([&](auto NI) {
+    constexpr int n = L - 1 - VQC_CI(NI);
+    static_for<L - n>([&](auto SI) {
+      constexpr int s = L - n - VQC_CI(SI);
+      static_for<s + 1>([&](auto TI) {
+        constexpr int t = VQC_CI(TI);
+        static_for<s - t + 1>([&](auto UI) {
+          constexpr int u = VQC_CI(UI), v = s - t - u;
           double val;
-          if (t > 0) {
+          if constexpr (t > 0) {
             val = X * R[tuv(t - 1, u, v)];
-            if (t > 1) val += (t - 1) * R[tuv(t - 2, u, v)];
-          } else if (u > 0) {
+            if constexpr (t > 1) val += (t - 1) * R[tuv(t - 2, u, v)];
+          } else if constexpr (u > 0) {
             val = Y * R[tuv(t, u - 1, v)];
-            if (u > 1) val += (u - 1) * R[tuv(t, u - 2, v)];
+            if constexpr (u > 1) val += (u - 1) * R[tuv(t, u - 2, v)];
           } else {
             val = Z * R[tuv(t, u, v - 1)];
-            if (v > 1) val += (v - 1) * R[tuv(t, u, v - 2)];
+            if constexpr (v > 1) val += (v - 1) * R[tuv(t, u, v - 2)];
           }
           R[tuv(t, u, v)] = val;
-        }
-      }
-    }
+        });
+      });
+    });
     R[0] = pw[n] * F[n];
-  }
+  });
 }
 
 // One-index Hermite coefficients E^{i,0}_t (i <= N) of (x-A)^i about P.
@@ -165,6 +212,100 @@ struct ThreeCenterArgs {
   double prim_cutoff;
 };
 
+// Ket contraction of one auxiliary primitive into Hermite bra space:
+// ws_tuv += pref * sum_k hc_k (2g)^-(LC+|k|)/2 R_{tuv + k}, t+u+v <= LBRA,
+// where k runs over the auxiliary function's Hermite terms (top degree only
+// for real-spherical functions, every degree LC, LC-2, ... for Cartesian).
+template <int LBRA, int LC, bool CART, int NH, int L>
+__device__ __forceinline__ void ket_contract(const double (&hc)[NH], double i2g, double pref, const double (&R)[ntuv(L)],
+                                             double (&ws)[ntuv(LBRA)]) {
+  double scale[LC + 1];
+  scale[0] = pref;
+#pragma unroll
+  for (int k = 1; k <= LC; ++k) scale[k] = scale[k - 1] * i2g;
+  static_for<ntuv(LBRA)>([&](auto EI) {
+    constexpr int e = VQC_CI(EI);
+    constexpr int s = tuv_degree(e), t = tuv_t(e), u = tuv_u(e), v = s - t - u;
+    double w = 0.0;
+    // Hermite terms of degree d = LC, LC - 2, ... (k counts them in order).
+    static_for<LC / 2 + 1>([&](auto DI) {
+      constexpr int d = LC - 2 * VQC_CI(DI);
+      if constexpr (CART || d == LC) {
+        constexpr int k0 = hermite_count(LC, true) - hermite_count(d, true);
+        static_for<nc(d)>([&](auto CI) {
+          constexpr int c = VQC_CI(CI);
+          constexpr int tt = cart_lx(d, c), uu = cart_ly(d, c), vv = cart_lz(d, c);
+          constexpr int k = k0 + tuv_pos(tt, uu, vv);
+          if constexpr (k < NH) w += hc[k] * scale[(LC + d) / 2] * R[tuv(t + tt, u + uu, v + vv)];
+        });
+      }
+    });
+    ws[e] += w;
+  });
+}
+
+// Bra contraction: out_e (e = Cartesian monomials of degree LO..HI, packed by
+// ne_offset) = sum_tuv Ex[ex][t] Ey[ey][u] Ez[ez][v] ws_tuv.
+template <int LO, int HI, int LBRA, class Fn>
+__device__ __forceinline__ void bra_contract(const double (&Ex)[LBRA + 1][LBRA + 1], const double (&Ey)[LBRA + 1][LBRA + 1],
+                                             const double (&Ez)[LBRA + 1][LBRA + 1], const double (&ws)[ntuv(LBRA)],
+                                             Fn&& emit) {
+  static_for<HI - LO + 1>([&](auto LI) {
+    constexpr int le = LO + VQC_CI(LI);
+    static_for<nc(le)>([&](auto CI) {
+      constexpr int c = VQC_CI(CI);
+      constexpr int ex = cart_lx(le, c), ey = cart_ly(le, c), ez = cart_lz(le, c);
+      double sum = 0.0;
+      static_for<ex + 1>([&](auto TI) {
+        constexpr int t = VQC_CI(TI);
+        double su = 0.0;
+        static_for<ey + 1>([&](auto UI) {
+          constexpr int u = VQC_CI(UI);
+          double sv = 0.0;
+          static_for<ez + 1>([&](auto VI) {
+            constexpr int v = VQC_CI(VI);
+            sv += Ez[ez][v] * ws[tuv(t, u, v)];
+          });
+          su += Ey[ey][u] * sv;
+        });
+        sum += Ex[ex][t] * su;
+      });
+      emit(std::integral_constant<int, le>{}, std::integral_constant<int, c>{}, sum);
+    });
+  });
+}
+
+// Horizontal recurrence from the (e0| accumulator (degrees LO..) to one
+// Cartesian (a b|: (x-B)^b = sum_k C(b,k) (x-A)^k (A-B)^(b-k).
+template <int LO, int NE, int AX, int AY, int AZ, int BX, int BY, int BZ>
+__device__ __forceinline__ double hrr_static(const double (&acc)[NE], double ABx, double ABy, double ABz) {
+  double v = 0.0;
+  static_for<BX + 1>([&](auto KXI) {
+    constexpr int kx = VQC_CI(KXI);
+    static_for<BY + 1>([&](auto KYI) {
+      constexpr int ky = VQC_CI(KYI);
+      static_for<BZ + 1>([&](auto KZI) {
+        constexpr int kz = VQC_CI(KZI);
+        double w = binom(BX, kx) * binom(BY, ky) * binom(BZ, kz);
+#pragma unroll
+        for (int r = 0; r < BX - kx; ++r) w *= ABx;
+#pragma unroll
+        for (int r = 0; r < BY - ky; ++r) w *= ABy;
+#pragma unroll
+        for (int r = 0; r < BZ - kz; ++r) w *= ABz;
+        constexpr int le = AX + AY + AZ + kx + ky + kz;
+        v += w * acc[ne_offset(LO, le) + cart_index(AX + kx, AY + ky, AZ + kz)];
+      });
+    });
+  });
+  return v;
+}
+
+// (ab|P) for one shell pair (blockIdx.x) and 64 auxiliary functions of one
+// angular momentum (one per lane).  Per bra primitive pair the auxiliary
+// primitives are summed in Hermite space (ws), then contracted with the bra's
+// Hermite coefficients once; every register index is a compile-time constant
+// (static_for), so nothing is kept in per-lane scratch memory.
 template <int LA, int LB, int LC, bool METRIC, bool CART>
 __global__ __launch_bounds__(64) void three_center_kernel(const ThreeCenterArgs a) {
   constexpr int LAB = LA + LB;
@@ -204,6 +345,9 @@ __global__ __launch_bounds__(64) void three_center_kernel(const ThreeCenterArgs 
       const double Px = (ea * A.x + eb * B.x) * ip;
       const double Py = (ea * A.y + eb * B.y) * ip;
       const double Pz = (ea * A.z + eb * B.z) * ip;
+      double ws[ntuv(LAB)];
+#pragma unroll
+      for (int k = 0; k < ntuv(LAB); ++k) ws[k] = 0.0;
       for (int ic = 0; ic < C.nprim; ++ic) {
         const double g = a.aux_exps[C.prim_off + ic];
         const double cg = a.aux_coefs[C.prim_off + ic];
@@ -214,111 +358,46 @@ __global__ __launch_bounds__(64) void three_center_kernel(const ThreeCenterArgs 
         boys_eval<L>(alpha * (X * X + Y * Y + Z * Z), a.boys, F);
         double R[ntuv(L)];
         hermite_r<L>(alpha, X, Y, Z, F, R);
-        // Ket: W_{tuv} = sum_k hc_k (2g)^{-(LC+s_k)/2} R_{t+tau_k, u+nu_k, v+phi_k}.
-        const double i2g = 0.5 / g;
-        double scale[LC + 1];
-        scale[0] = 1.0;
-#pragma unroll
-        for (int k = 1; k <= LC; ++k) scale[k] = scale[k - 1] * i2g;
-        double W[ntuv(LAB)];
-#pragma unroll
-        for (int s = 0; s <= LAB; ++s)
-#pragma unroll
-          for (int t = 0; t <= s; ++t)
-#pragma unroll
-            for (int u = 0; u <= s - t; ++u) {
-              const int v = s - t - u;
-              double w = 0.0;
-              int k = 0;
-#pragma unroll
-              for (int d = LC; d >= 0; d -= 2) {
-#pragma unroll
-                for (int tt = 0; tt <= d; ++tt)
-#pragma unroll
-                  for (int uu = 0; uu <= d - tt; ++uu) {
-                    const int vv = d - tt - uu;
-                    if (k < NH) w += hc[k] * scale[(LC + d) / 2] * R[tuv(t + tt, u + uu, v + vv)];
-                    ++k;
-                  }
-                if (!CART) break;
-              }
-              W[tuv(t, u, v)] = w;
-            }
-        const double pref = 2.0 * 17.493418327624862 /* pi^{5/2} */ / (p * g * sqrt(q)) * kab * cg;
-        double Ex[LAB + 1][LAB + 1], Ey[LAB + 1][LAB + 1], Ez[LAB + 1][LAB + 1];
-        hermite_e0<LAB>(Px - A.x, 0.5 * ip, Ex);
-        hermite_e0<LAB>(Py - A.y, 0.5 * ip, Ey);
-        hermite_e0<LAB>(Pz - A.z, 0.5 * ip, Ez);
-#pragma unroll
-        for (int le = LA; le <= LAB; ++le)
-#pragma unroll
-          for (int c = 0; c < nc(le); ++c) {
-            const int ex = cart_lx(le, c), ey = cart_ly(le, c), ez = cart_lz(le, c);
-            double s = 0.0;
-#pragma unroll
-            for (int t = 0; t <= ex; ++t) {
-              double su = 0.0;
-#pragma unroll
-              for (int u = 0; u <= ey; ++u) {
-                double sv = 0.0;
-#pragma unroll
-                for (int v = 0; v <= ez; ++v) sv += Ez[ez][v] * W[tuv(t, u, v)];
-                su += Ey[ey][u] * sv;
-              }
-              s += Ex[ex][t] * su;
-            }
-            acc[ne_offset(LA, le) + c] += pref * s;
-          }
+        const double pref = 2.0 * 17.493418327624862 /* pi^{5/2} */ / (p * g * sqrt(q)) * cg;
+        ket_contract<LAB, LC, CART, NH, L>(hc, 0.5 / g, pref, R, ws);
       }
+      double Ex[LAB + 1][LAB + 1], Ey[LAB + 1][LAB + 1], Ez[LAB + 1][LAB + 1];
+      hermite_e0<LAB>(Px - A.x, 0.5 * ip, Ex);
+      hermite_e0<LAB>(Py - A.y, 0.5 * ip, Ey);
+      hermite_e0<LAB>(Pz - A.z, 0.5 * ip, Ez);
+      bra_contract<LA, LAB, LAB>(Ex, Ey, Ez, ws, [&](auto LI, auto CI, double v) {
+        acc[ne_offset(LA, VQC_CI(LI)) + VQC_CI(CI)] += kab * v;
+      });
     }
   }
 
-  // Horizontal recurrence to Cartesian (ab|, then public AOs.
+  // Horizontal recurrence to Cartesian (ab|, then public AOs.  The transform
+  // rows past nfa / nfb are zero padding, so the loops have constant trip
+  // counts; only the stores test the public counts.
   double cart[NA][NB];
-#pragma unroll
-  for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-    for (int ib = 0; ib < NB; ++ib) {
-      const int ax = cart_lx(LA, ia), ay = cart_ly(LA, ia), az = cart_lz(LA, ia);
-      const int bx = cart_lx(LB, ib), by = cart_ly(LB, ib), bz = cart_lz(LB, ib);
-      double v = 0.0;
-#pragma unroll
-      for (int kx = 0; kx <= bx; ++kx)
-#pragma unroll
-        for (int ky = 0; ky <= by; ++ky)
-#pragma unroll
-          for (int kz = 0; kz <= bz; ++kz) {
-            double w = binom(bx, kx) * binom(by, ky) * binom(bz, kz);
-#pragma unroll
-            for (int r = 0; r < bx - kx; ++r) w *= ABx;
-#pragma unroll
-            for (int r = 0; r < by - ky; ++r) w *= ABy;
-#pragma unroll
-            for (int r = 0; r < bz - kz; ++r) w *= ABz;
-            const int le = LA + kx + ky + kz;
-            v += w * acc[ne_offset(LA, le) + cart_index(ax + kx, ay + ky, az + kz)];
-          }
-      cart[ia][ib] = v;
-    }
-
+  static_for<NA>([&](auto IA) {
+    constexpr int ia = VQC_CI(IA);
+    static_for<NB>([&](auto IB) {
+      constexpr int ib = VQC_CI(IB);
+      cart[ia][ib] = hrr_static<LA, NE, cart_lx(LA, ia), cart_ly(LA, ia), cart_lz(LA, ia), cart_lx(LB, ib),
+                                cart_ly(LB, ib), cart_lz(LB, ib)>(acc, ABx, ABy, ABz);
+    });
+  });
   double half[NA][NB];  // [public a][cartesian b]
 #pragma unroll
   for (int ma = 0; ma < NA; ++ma)
 #pragma unroll
     for (int ib = 0; ib < NB; ++ib) {
       double v = 0.0;
-      if (ma < a.nfa) {
 #pragma unroll
-        for (int ia = 0; ia < NA; ++ia) v += a.ta[ma * NA + ia] * cart[ia][ib];
-      }
+      for (int ia = 0; ia < NA; ++ia) v += a.ta[ma * NA + ia] * cart[ia][ib];
       half[ma][ib] = v;
     }
 #pragma unroll
   for (int ma = 0; ma < NA; ++ma) {
-    if (ma >= a.nfa) break;
 #pragma unroll
     for (int mb = 0; mb < NB; ++mb) {
-      if (mb >= a.nfb) break;
+      if (ma >= a.nfa || mb >= a.nfb) continue;
       double v = 0.0;
 #pragma unroll
       for (int ib = 0; ib < NB; ++ib) v += a.tb[mb * NB + ib] * half[ma][ib];

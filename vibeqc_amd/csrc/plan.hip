@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <limits>
 #include <numeric>
 #include <stdexcept>
@@ -152,13 +153,40 @@ void DfPlan::setup(const System& aux) {
     sync();
   }
   trace.mark("one_electron");
+  // VIBEQC_TRACE=checksum: fixed-order sums of squares of the setup's
+  // intermediate and final tensors, keyed by the nuclear repulsion (a
+  // debugging aid that shows whether two runs built identical tensors).
+  const char* trace_env = std::getenv("VIBEQC_TRACE");
+  const bool checksums = trace_env && std::strcmp(trace_env, "checksum") == 0;
+  auto sum_squares = [&](const double* x, std::size_t count) {
+    DeviceArray<double> sums(2);
+    dot_pair(x, x, x, count, sums.data(), stream_);
+    double h[2];
+    sums.download(h, 2, stream_);
+    sync();
+    return h[0];
+  };
+  // Device copies of the inputs (uploaded above): Boys table and both bases.
+  const double sum_in = checksums ? sum_squares(boys_.data(), boys_.size()) + sum_squares(al.basis.exps.data(), al.basis.exps.size()) +
+                                        sum_squares(al.basis.coefs.data(), al.basis.coefs.size()) +
+                                        sum_squares(al.hermite.data(), al.hermite.size()) +
+                                        sum_squares(ob.exps.data(), ob.exps.size()) + sum_squares(ob.coefs.data(), ob.coefs.size())
+                                  : 0.0;
   DeviceArray<double> M(static_cast<std::size_t>(naux_) * naux_);
   compute_metric(al, boys_.data(), M.data(), stream_);
   sync();
+  const double sum_m = checksums ? sum_squares(M.data(), M.size()) : 0.0;
+  const double sum_in2 = checksums ? sum_squares(boys_.data(), boys_.size()) + sum_squares(al.basis.exps.data(), al.basis.exps.size()) +
+                                         sum_squares(al.basis.coefs.data(), al.basis.coefs.size()) +
+                                         sum_squares(al.hermite.data(), al.hermite.size()) +
+                                         sum_squares(ob.exps.data(), ob.exps.size()) + sum_squares(ob.coefs.data(), ob.coefs.size())
+                                   : 0.0;
+  const double sum_m2 = checksums ? sum_squares(M.data(), M.size()) : 0.0;
   trace.mark("metric");
   B64_.resize(npair_ * ldq_);
   compute_three_center(orb_, ob, al, boys_.data(), B64_.data(), ldq_, opts_.screening * 1e-10, stream_);
   sync();
+  const double sum_a = checksums ? sum_squares(B64_.data(), B64_.size()) : 0.0;
   trace.mark("three_center");
   factor_metric(M, B64_);
   trace.mark(cholesky_ ? "cholesky_trsm" : "eigen_cutoff_gemm");
@@ -170,6 +198,13 @@ void DfPlan::setup(const System& aux) {
   trace.mark("finite_check");
   trace.finish(n_, naux_, npair_);
   if (bad) throw NumericalFailure("non-finite fitted three-centre tensor");
+  if (checksums)
+    std::fprintf(stderr,
+                 "{\"trace\": \"setup_checksum\", \"enuc\": %.17g, \"B\": %.17g, \"S\": %.17g, \"H\": %.17g, "
+                 "\"L\": %.17g, \"M\": %.17g, \"A\": %.17g, \"cholesky\": %d, \"IN\": %.17g, \"IN2\": %.17g, \"M2\": %.17g}\n",
+                 enuc_, sum_squares(B64_.data(), B64_.size()), sum_squares(S_.data(), nn), sum_squares(H_.data(), nn),
+                 L_.size() ? sum_squares(L_.data(), L_.size()) : 0.0, sum_m, sum_a, cholesky_ ? 1 : 0, sum_in, sum_in2,
+                 sum_m2);
 }
 
 void DfPlan::factor_metric(DeviceArray<double>& M, DeviceArray<double>& A) {
