@@ -178,6 +178,21 @@ GENERATIVEQC_API generativeqc_status vibeqc_fock_plan_evaluate_occupied(
     generativeqc_fock_plan* plan, const double* occupied, uint64_t nbf, uint64_t nocc,
     double* coulomb, double* exchange);
 
+/* Extension with no counterpart in the reference, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): vibeqc_fock_plan_evaluate_occupied with a
+ * device-resident factor (row-major nbf x nocc, 1 <= nocc <= nbf) and
+ * device-resident nbf^2 outputs, at the plan's precision (a reduced precision
+ * must have been selected by vibeqc_fock_plan_set_precision, which makes its
+ * copy of B).  Every workspace lives in one fixed 128 MiB array per device
+ * (held until the process ends); no device or host memory is allocated.
+ * Calls whose workspace would exceed it return INVALID_ARGUMENT.  All calls on
+ * one device share that array: calls that overlap on one device race (the
+ * caller serialises them).  Returns after J and K are written. */
+GENERATIVEQC_API generativeqc_status vibeqc_fock_plan_evaluate_occupied_noalloc(
+    generativeqc_fock_plan* plan, const double* occupied_device, uint64_t nbf, uint64_t nocc,
+    double* coulomb_device, double* exchange_device);
+
 /* Device-resident timing harness: uploads C once (as above), runs one
  * untimed build, synchronises, then runs `repeat` complete J+K builds on the
  * device with no host transfer in between and synchronises again.

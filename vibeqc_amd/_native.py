@@ -331,6 +331,10 @@ EXPORTS = {
     "generativeqc_fock_plan_diagnostic": (ct.c_int32, [ct.c_void_p, ct.POINTER(FockDiagnosticC)]),
     "vibeqc_fock_plan_set_precision": (ct.c_int32, [ct.c_void_p, ct.c_int32]),
     "vibeqc_fock_plan_evaluate_occupied": (ct.c_int32, [ct.c_void_p, _DP, ct.c_uint64, ct.c_uint64, _DP, _DP]),
+    "vibeqc_fock_plan_evaluate_occupied_noalloc": (
+        ct.c_int32,
+        [ct.c_void_p, ct.c_void_p, ct.c_uint64, ct.c_uint64, ct.c_void_p, ct.c_void_p],
+    ),
     "vibeqc_fock_plan_benchmark_occupied": (
         ct.c_int32,
         [ct.c_void_p, _DP, ct.c_uint64, ct.c_uint64, ct.c_uint32, _DP],

@@ -28,7 +28,7 @@ SOURCES = (
     ("integrals_g22.hip", "integrals_g21.hip")
     + tuple(reversed(CLASS_UNITS))
     + tuple(u for u in reversed(GRAD_UNITS) if u not in ("integrals_g22.hip", "integrals_g21.hip"))
-    + ("integrals.hip", "integrals_grad.hip", "jk.hip", "plan.hip", "uhf.hip", "gradient.hip", "scf.hip", "basis.cpp", "batch.cpp", "capi.cpp")
+    + ("integrals.hip", "integrals_grad.hip", "jk.hip", "noalloc.hip", "plan.hip", "uhf.hip", "gradient.hip", "scf.hip", "basis.cpp", "batch.cpp", "capi.cpp")
 )
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 

@@ -22,6 +22,7 @@
 #include "generativeqc/calculation.h"
 #include "generativeqc/fock.h"
 #include "generativeqc/generativeqc.h"
+#include "noalloc.hpp"
 #include "plan.hpp"
 
 struct generativeqc_context {
@@ -562,6 +563,33 @@ generativeqc_status vibeqc_fock_plan_evaluate_occupied(generativeqc_fock_plan* p
     if (coulomb) std::copy(jk.J.begin(), jk.J.end(), coulomb);
     if (exchange) std::copy(jk.K.begin(), jk.K.end(), exchange);
     return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(&plan->detail);
+  }
+}
+
+generativeqc_status vibeqc_fock_plan_evaluate_occupied_noalloc(generativeqc_fock_plan* plan,
+                                                               const double* occupied_device, uint64_t nbf,
+                                                               uint64_t nocc, double* coulomb_device,
+                                                               double* exchange_device) {
+  if (!plan || !occupied_device || !coulomb_device || !exchange_device) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  plan->detail.clear();
+  try {
+    auto& p = *plan->plan;
+    if (nbf != static_cast<uint64_t>(p.nbf()) || nocc == 0 || nocc > nbf) {
+      plan->detail = "occupied factor shape mismatch";
+      return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+    }
+    if (coulomb_device == exchange_device) {
+      plan->detail = "Fock output buffers overlap";
+      return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+    }
+    p.evaluate_occupied_into(occupied_device, static_cast<int>(nocc), coulomb_device, exchange_device,
+                             vqc::noalloc_scratch(p.device()), vqc::kNoAllocScratchBytes);
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (const std::length_error& e) {
+    plan->detail = e.what();
+    return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
   } catch (...) {
     return map_exception(&plan->detail);
   }

@@ -1153,9 +1153,16 @@ std::size_t u_row_stride(Precision prec, int r, int ldq) {
 }
 
 void exchange_syrk(Precision prec, const void* U, std::size_t ustride, int n, int r, int r_pos, int ldq, double* K,
-                   double* partial, hipStream_t s) {
+                   double* partial, hipStream_t s, std::size_t partial_capacity) {
   const std::size_t kd = static_cast<std::size_t>(r) * ldq;
   if (ustride < kd) throw std::invalid_argument("U row stride shorter than r x ldq");
+  // Slices per run: both runs (positive and negative columns) together must
+  // fit the partial buffer (n^2 doubles per FP64 slice; FP32 slices use half).
+  const std::size_t slice = static_cast<std::size_t>(n) * n;
+  const int cap = partial_capacity == 0
+                      ? syrk_split_cap(n)
+                      : static_cast<int>(std::min<std::size_t>(syrk_split_cap(n), partial_capacity / (2 * slice)));
+  if (cap < 1) throw std::length_error("exchange partial buffer holds no slice");
   int slices = 0;
   auto run = [&](std::size_t kb, std::size_t ke, double sign) {
     if (ke <= kb) return;
@@ -1171,7 +1178,7 @@ void exchange_syrk(Precision prec, const void* U, std::size_t ustride, int n, in
     // Resident 512-thread blocks on 256 CUs: one per CU for BF16X3 (80 KB LDS)
     // and FP64 (102 KB), two for FP32 (51 KB).
     const int slots = prec == Precision::FP32 ? 512 : 256;
-    int splits = syrk_splits(tiles, syrk_split_cap(n), slots);
+    int splits = syrk_splits(tiles, cap, slots);
     const std::size_t chunk = round_up((len + splits - 1) / splits, bk);
     splits = static_cast<int>((len + chunk - 1) / chunk);
     const int nwork = tiles * splits;
@@ -1204,6 +1211,37 @@ void exchange_syrk(Precision prec, const void* U, std::size_t ustride, int n, in
     hipLaunchKernelGGL(reduce_exchange_kernel<float>, dim3((e + 255) / 256), dim3(256), 0, s,
                        reinterpret_cast<const float*>(partial), slices, n, K);
   VQC_LAUNCH("reduce_exchange_kernel");
+}
+
+namespace {
+__global__ void occupied_factor_kernel(const double* __restrict__ C, int n, int nocc, int ldc,
+                                       double* __restrict__ factor) {
+  const std::size_t e = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<std::size_t>(n) * ldc) return;
+  const int nu = static_cast<int>(e / ldc), i = static_cast<int>(e % ldc);
+  factor[e] = i < nocc ? 1.4142135623730951 * C[static_cast<std::size_t>(nu) * nocc + i] : 0.0;
+}
+
+// D[mu][nu] = sum_i F[mu][i] F[nu][i], one element per thread (fixed order).
+__global__ void factor_product_kernel(const double* __restrict__ F, int n, int nocc, int ldc, double* __restrict__ D) {
+  const std::size_t e = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<std::size_t>(n) * n) return;
+  const int mu = static_cast<int>(e / n), nu = static_cast<int>(e % n);
+  const double* a = F + static_cast<std::size_t>(mu) * ldc;
+  const double* b = F + static_cast<std::size_t>(nu) * ldc;
+  double s = 0.0;
+  for (int i = 0; i < nocc; ++i) s += a[i] * b[i];
+  D[e] = s;
+}
+}  // namespace
+
+void occupied_factor_and_density(const double* C, int n, int nocc, int ldc, double* factor, double* D,
+                                 hipStream_t s) {
+  const std::size_t ef = static_cast<std::size_t>(n) * ldc, ed = static_cast<std::size_t>(n) * n;
+  hipLaunchKernelGGL(occupied_factor_kernel, dim3((ef + 255) / 256), dim3(256), 0, s, C, n, nocc, ldc, factor);
+  VQC_LAUNCH("occupied_factor_kernel");
+  hipLaunchKernelGGL(factor_product_kernel, dim3((ed + 255) / 256), dim3(256), 0, s, factor, n, nocc, ldc, D);
+  VQC_LAUNCH("factor_product_kernel");
 }
 
 void convert_f64_to_f32(const double* in, float* out, std::size_t count, hipStream_t s) {

@@ -62,9 +62,15 @@ void occupied_u(Precision prec, const void* B, const OccupiedOperands& ops, int 
 // K (n x n, FP64, both triangles) = sum_k sign_k W[mu][k] W[lam][k] where
 // W[mu] = U[mu] (r ldq elements, rows ustride apart) and columns i < r_pos
 // carry +1, the rest -1.  `partial` needs exchange_partial_size(n) doubles.
+// `partial_capacity` (doubles) bounds the split-K slices to what `partial`
+// holds (0: exchange_partial_size(n)).
 std::size_t exchange_partial_size(int n);
 void exchange_syrk(Precision prec, const void* U, std::size_t ustride, int n, int r, int r_pos, int ldq, double* K,
-                   double* partial, hipStream_t s);
+                   double* partial, hipStream_t s, std::size_t partial_capacity = 0);
+// C (row-major n x nocc) -> factor n x ldc scaled by sqrt(2), zero padded, and
+// D = (that factor)(that factor)^T = 2 C C^T (n x n); both fully written.
+void occupied_factor_and_density(const double* C, int n, int nocc, int ldc, double* factor, double* D,
+                                 hipStream_t s);
 
 // ---- conversions ---------------------------------------------------------
 void convert_f64_to_f32(const double* in, float* out, std::size_t count, hipStream_t s);

@@ -479,6 +479,64 @@ JkResult DfPlan::evaluate_occupied(const std::vector<double>& occupied, int nocc
   return out;
 }
 
+void DfPlan::evaluate_occupied_into(const double* C, int nocc, double* J, double* K, void* scratch,
+                                    std::size_t bytes) {
+  VQC_HIP(hipSetDevice(device_));
+  converged_state_ = false;
+  if (nocc < 1 || nocc > n_) throw std::invalid_argument("occupied column count out of range");
+  if (!C || !J || !K || !scratch) throw std::invalid_argument("null device buffer");
+  const Precision p = prec_;
+  if (p != Precision::FP64 && !B32_.size())
+    throw std::logic_error("the reduced-precision copy of B is made by set_precision");
+  // Workspace carved from the scratch block, each piece 256-byte aligned.
+  std::size_t used = 0;
+  auto carve = [&](std::size_t size) {
+    used = round_up(used, 256);
+    if (size > bytes || used > bytes - size)
+      throw std::length_error("workspace exceeds the " + std::to_string(bytes >> 20) + " MiB allocation-free scratch");
+    void* at = static_cast<char*>(scratch) + used;
+    used += size;
+    return at;
+  };
+  OccupiedOperands ops;
+  ops.n = n_;
+  ops.r = nocc;
+  ops.ni = occupied_tile(nocc);
+  ops.ldc = static_cast<int>(round_up(nocc, 32 * ops.ni));
+  ops.ldct = static_cast<int>(round_up(n_, 32));
+  ops.ustride = u_row_stride(p, nocc, ldq_);
+  const std::size_t nn = static_cast<std::size_t>(n_) * n_;
+  double* c64 = static_cast<double*>(carve(static_cast<std::size_t>(n_) * ops.ldc * sizeof(double)));
+  ops.c64 = c64;
+  if (p == Precision::FP32) ops.c32 = static_cast<float*>(carve(static_cast<std::size_t>(n_) * ops.ldc * sizeof(float)));
+  if (p == Precision::BF16X3) {
+    const std::size_t plane = static_cast<std::size_t>(ops.ldc) * ops.ldct * sizeof(uint16_t);
+    ops.ct_hi = carve(plane);
+    ops.ct_lo = carve(plane);
+  }
+  double* D = static_cast<double*>(carve(nn * sizeof(double)));
+  double* rho_part = static_cast<double*>(carve(static_cast<std::size_t>(n_) * ldq_ * sizeof(double)));
+  double* rho = static_cast<double*>(carve(static_cast<std::size_t>(ldq_) * sizeof(double)));
+  const std::size_t u_elem = p == Precision::FP64 ? sizeof(double) : sizeof(float);  // BF16X3: two 2-byte planes
+  void* U = carve(static_cast<std::size_t>(n_) * ops.ustride * u_elem);
+  // The split-K partials take what is left (at least the two slices of one split).
+  used = round_up(used, 256);
+  if (used > bytes || (bytes - used) / sizeof(double) < 2 * nn)
+    throw std::length_error("workspace exceeds the " + std::to_string(bytes >> 20) + " MiB allocation-free scratch");
+  double* partial = static_cast<double*>(carve((bytes - used) / sizeof(double) * sizeof(double)));
+  const std::size_t partial_capacity = std::min((bytes - (reinterpret_cast<char*>(partial) - static_cast<char*>(scratch))) /
+                                                    sizeof(double),
+                                                exchange_partial_size(n_));
+  const void* B = p == Precision::FP64 ? static_cast<const void*>(B64_.data()) : static_cast<const void*>(B32_.data());
+  occupied_factor_and_density(C, n_, nocc, ops.ldc, c64, D, stream_);
+  prepare_occupied(p, ops, stream_);
+  occupied_u(p, B, ops, ldq_, U, D, rho_part, stream_);
+  exchange_syrk(p, U, ops.ustride, n_, nocc, nocc, ldq_, K, partial, stream_, partial_capacity);
+  rho_from_partials(rho_part, n_, ldq_, rho, stream_);
+  coulomb_matrix(p, B, rho, npair_, ldq_, n_, J, stream_);
+  sync();
+}
+
 std::array<double, 5> DfPlan::benchmark_occupied(const std::vector<double>& occupied, int nocc, unsigned repeat) {
   converged_state_ = false;
   VQC_HIP(hipSetDevice(device_));

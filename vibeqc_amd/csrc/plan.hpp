@@ -64,6 +64,7 @@ class DfPlan {
   DfPlan(const DfPlan&) = delete;
   DfPlan& operator=(const DfPlan&) = delete;
 
+  int device() const { return device_; }
   int nbf() const { return n_; }
   int naux() const { return naux_; }
   std::size_t npair() const { return npair_; }
@@ -87,6 +88,13 @@ class DfPlan {
   JkResult evaluate(const std::vector<double>& density, std::vector<double>* gradient = nullptr);
   // Raw J/K for D = 2 C C^T (C row-major n x nocc, host).
   JkResult evaluate_occupied(const std::vector<double>& occupied, int nocc);
+  // Raw J/K for D = 2 C C^T from a device-resident occupied factor C
+  // (row-major n x nocc) into device outputs J and K (n x n), at the plan's
+  // precision, with every workspace carved from `scratch` (`bytes` long) and
+  // no device or host allocation; returns after J and K are written.  A
+  // reduced precision needs its B copy, made by set_precision.  Throws
+  // std::length_error when the workspace does not fit.
+  void evaluate_occupied_into(const double* C, int nocc, double* J, double* K, void* scratch, std::size_t bytes);
   // Timing harness (see vibeqc_fock_plan_benchmark_occupied).
   std::array<double, 5> benchmark_occupied(const std::vector<double>& occupied, int nocc, unsigned repeat);
 
