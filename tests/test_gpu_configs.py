@@ -190,3 +190,23 @@ def test_water64_full_size():
     assert abs(rmx.energy - r64.energy) < 1e-6
     assert np.abs(fmx - f64).max() < 1e-6
     assert np.abs(f64.sum(axis=0)).max() < 1e-8 and np.abs(fmx.sum(axis=0)).max() < 1e-8
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("waters,nocc", [(24, 120), (36, 180)])
+def test_mid_size_occupied_tiles(waters, nocc):
+    """FP32 and BF16X3 J/K against FP64 for occupied widths between the
+    C20H42 and (H2O)64 shapes (i tiles of 4 to 6 MFMA column blocks)."""
+    from vibeqc_amd.geometry import lowdin_occupied, water_cluster
+
+    case = molecule_case(f"water{waters}", water_cluster(waters, 12.4 * (waters / 64) ** (1 / 3), seed=9))
+    with _plan(case) as plan:
+        S, _ = plan.one_electron()
+        C = lowdin_occupied(S, nocc, seed=3)
+        j64, k64 = plan.evaluate_occupied(C)
+        for precision, tol_k in (("fp32", 2e-4), ("bf16x3", 5e-5)):
+            plan.set_precision(precision)
+            j, k = plan.evaluate_occupied(C)
+            print(f"(H2O){waters} nocc {nocc} {precision}: rel J {_rel(j, j64):.2e}, rel K {_rel(k, k64):.2e}")
+            assert _rel(j, j64) < 1e-6 and _rel(k, k64) < tol_k
+

@@ -533,7 +533,7 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
 // global loads of the next step are issued before its B gathers, so the wait
 // that precedes their LDS store never waits for the gathers.
 template <int NIW>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void occupied_u_bf16x3_direct_kernel(
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NIW <= 3 ? 4 : 2))) void occupied_u_bf16x3_direct_kernel(
     const float* __restrict__ B, const __bf16* __restrict__ Cth, const __bf16* __restrict__ Ctl,
     __bf16* __restrict__ Uh, __bf16* __restrict__ Ul, std::size_t us, int n, int ldq, int ldct, int r,
     const double* __restrict__ D, double* __restrict__ rho_part) {
@@ -984,7 +984,7 @@ void launch_u(Precision prec, const void* B, const OccupiedOperands& ops, int ld
     dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (32 * NIW));
     hipLaunchKernelGGL((occupied_u_kernel<float, NIW, WQ>), grid, dim3(128 * WQ), dyn, s, static_cast<const float*>(B),
                        ops.c32, static_cast<float*>(U), ops.ustride, ops.n, ldq, ops.ldc, ops.r, nqb, D, rho_part);
-  } else if constexpr (NIW <= 3) {
+  } else if constexpr (NIW <= 10) {
     const int nqb = (ldq + 255) / 256;
     dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (32 * NIW));
     // Row-offset table (n + 32 entries) and the rho row as float.
