@@ -45,10 +45,38 @@ def test_posthf_fixture_fock_and_energy(oracle, name):
         assert abs(r.energy - meta["df_hf_energy"]) < 1e-9
 
 
-def test_orbital_f_shells_report_not_implemented():
+def test_orbital_f_shells_against_fixture_and_oracle(oracle):
+    """HeH+ with an orbital f shell fitted by itself (PySCF posthf f_heh): J/K
+    against the oracle (1e-10), the fixture's Fock matrix (1e-10) and DF-RHF
+    energy (1e-9 Eh), and forces against the oracle's analytic gradient, which
+    central differences pin (1e-8 Eh/bohr)."""
+    case, meta, ref = posthf_case("f_heh")
+    assert max(s.l for s in case.shells) == 3
+    P = case.oracle_problem(oracle)
+    C, occ = ref["df_C"], ref["df_occ"]
+    D = (C * occ) @ C.T
+    J, K = oracle.jk(P.B, D)
+    tight = dict(energy_tolerance=1e-12, density_tolerance=1e-10)
+    with _plan(case) as plan:
+        ev = plan.evaluate(D)
+        r = plan.solve(compute_forces=True, **tight)
+    assert np.abs(ev.coulomb - J).max() < 1e-10 and np.abs(ev.exchange - K).max() < 1e-10
+    assert np.abs(ev.fock - ref["df_F"]).max() < 1e-10
+    assert abs(r.energy - meta["df_hf_energy"]) < 1e-9
+    sh, ax = case.oracle_shells(oracle), case.oracle_shells(oracle, True)
+    g = oracle.df_rhf_gradient(case.z, case.xyz, sh, case.spherical, ax, case.spherical, P, oracle.df_rhf(P, **tight))
+    print(f"f_heh: dE {r.energy - meta['df_hf_energy']:.2e}, max |F + g_oracle| {np.abs(np.asarray(r.forces) + g).max():.2e}")
+    assert np.abs(np.asarray(r.forces) + g).max() < 1e-8
+
+
+def test_orbital_g_shells_report_not_implemented():
+    from vibeqc_amd.basis import AOBasis, Primitive, Shell
+
     case, _, _ = posthf_case("f_heh")
+    shells = case.shells + (Shell(0, 4, (Primitive(1.2, 1.0),)),)
     with pytest.raises(_native.NativeError) as err:
-        _plan(case)
+        FockPlan(AOBasis(case.atoms, shells, case.representation, case.charge, case.multiplicity),
+                 auxiliary=case.auxiliary())
     assert err.value.status == 3
 
 

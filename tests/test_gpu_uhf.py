@@ -255,3 +255,28 @@ def test_mixed_precision_uhf_forces():
     print(f"dE {mixed.energy - e64.energy:.2e} dF {np.abs(np.asarray(mixed.forces) - np.asarray(e64.forces)).max():.2e}")
     assert abs(mixed.energy - e64.energy) < 1e-6
     assert np.abs(np.asarray(mixed.forces) - np.asarray(e64.forces)).max() < 1e-6
+
+
+def test_uhf_batch_with_forces_eight_radicals():
+    """Eight OH doublets (def2-SVP / cc-pVDZ-JKFIT, d shells on O) as one UHF
+    batch with forces, executed three times in one process: every execution
+    succeeds and each member's energy and forces equal the single-plan ones
+    (1e-9 Eh, 1e-8 Eh/bohr); repeated executions agree bitwise."""
+    from vibeqc_amd import Calculator
+    from vibeqc_amd.basis import Atom
+
+    systems = [(Atom(8, (0.0, 0.0, 0.0)), Atom(1, (0.0, 0.03 * k, 1.76 + 0.02 * k))) for k in range(8)]
+    calc = Calculator(method="uhf", basis="def2-svp")
+    runs = []
+    with calc.prepare_batch(systems, multiplicities=[2] * 8, warm_start=False) as batch:
+        for _ in range(3):
+            res = batch.execute(strict=True)
+            runs.append((np.asarray(res.energies), np.stack([np.asarray(it.forces) for it in res.items])))
+    for e, f in runs[1:]:
+        assert np.array_equal(e, runs[0][0]) and np.array_equal(f, runs[0][1])
+    for k, atoms in enumerate(systems):
+        single = calc.singlepoint(atoms, multiplicity=2)
+        de = runs[0][0][k] - single.energy
+        df = np.abs(runs[0][1][k] - np.asarray(single.forces)).max()
+        print(f"OH {k}: E {single.energy:.10f} dE {de:.1e} dF {df:.1e}")
+        assert abs(de) < 1e-9 and df < 1e-8

@@ -21,13 +21,12 @@ ROOT = PACKAGE.parent
 CSRC = PACKAGE / "csrc"
 BUILD = PACKAGE / "_build"
 LIBRARY = PACKAGE / "libvibeqc_amd.so"
-CLASS_UNITS = tuple(f"integrals_c{a}{b}.hip" for a in range(3) for b in range(a + 1)) + ("integrals_c30.hip",)
-GRAD_UNITS = tuple(f"integrals_g{a}{b}.hip" for a in range(3) for b in range(a + 1)) + ("integrals_g30.hip",)
+CLASS_UNITS = tuple(f"integrals_c{a}{b}.hip" for a in range(4) for b in range(a + 1))
+GRAD_UNITS = tuple(f"integrals_g{a}{b}.hip" for a in range(4) for b in range(a + 1))
 # Largest translation units first so the parallel build finishes soonest.
 SOURCES = (
-    ("integrals_g22.hip", "integrals_g21.hip")
+    tuple(reversed(GRAD_UNITS))
     + tuple(reversed(CLASS_UNITS))
-    + tuple(u for u in reversed(GRAD_UNITS) if u not in ("integrals_g22.hip", "integrals_g21.hip"))
     + ("integrals.hip", "integrals_grad.hip", "jk.hip", "noalloc.hip", "plan.hip", "uhf.hip", "gradient.hip", "scf.hip", "basis.cpp", "batch.cpp", "capi.cpp")
 )
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))

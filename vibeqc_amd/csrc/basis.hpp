@@ -11,7 +11,7 @@
 
 namespace vqc {
 
-constexpr int kMaxOrbitalL = 2;    // kernels are instantiated for s, p, d orbital shells
+constexpr int kMaxOrbitalL = 3;    // kernels are instantiated for s..f orbital shells
 constexpr int kMaxAuxiliaryL = 3;  // and s..f auxiliary shells
 constexpr int kMaxPublicL = 4;     // normalisation/transforms are defined through g
 

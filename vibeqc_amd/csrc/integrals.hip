@@ -36,7 +36,11 @@ void dispatch_pair(int la, int lb, int lc, bool cart, const ThreeCenterArgs& arg
     case 20: dispatch_3c_20(lc, cart, args, ntask, s); break;
     case 21: dispatch_3c_21(lc, cart, args, ntask, s); break;
     case 22: dispatch_3c_22(lc, cart, args, ntask, s); break;
-    default: throw NotImplemented("orbital angular momentum above d");
+    case 30: dispatch_3c_30(lc, cart, args, ntask, s); break;
+    case 31: dispatch_3c_31(lc, cart, args, ntask, s); break;
+    case 32: dispatch_3c_32(lc, cart, args, ntask, s); break;
+    case 33: dispatch_3c_33(lc, cart, args, ntask, s); break;
+    default: throw NotImplemented("orbital angular momentum above f");
   }
 }
 
@@ -58,7 +62,11 @@ void dispatch_1e(int la, int lb, const OneElectronArgs& args, hipStream_t s) {
     case 20: dispatch_1e_20(args, s); break;
     case 21: dispatch_1e_21(args, s); break;
     case 22: dispatch_1e_22(args, s); break;
-    default: throw NotImplemented("orbital angular momentum above d");
+    case 30: dispatch_1e_30(args, s); break;
+    case 31: dispatch_1e_31(args, s); break;
+    case 32: dispatch_1e_32(args, s); break;
+    case 33: dispatch_1e_33(args, s); break;
+    default: throw NotImplemented("orbital angular momentum above f");
   }
 }
 
@@ -232,7 +240,7 @@ void compute_one_electron(const System& sys, const DeviceBasis& basis, const dou
     args.T = T;
     args.V = V;
     args.n = static_cast<int>(basis.nfunctions);
-    args.sub = std::max(2, one_electron_lanes(args.natom));
+    args.sub = std::max(2, one_electron_lanes(args.natom, one_electron_pair_bytes(la, lb)));
     dispatch_1e(la, lb, args, s);
   }
   VQC_HIP(hipStreamSynchronize(s));

@@ -1,9 +1,11 @@
-// Nuclear-derivative kernels of the auxiliary metric class l=3; compiled as
-// its own translation unit so the classes build in parallel.
+// Nuclear-derivative kernels of the orbital class (l=3, l=0) and of the
+// auxiliary metric class l=3; compiled as its own translation unit so the
+// classes build in parallel.
 #include "integrals_grad.hpp"
 
 namespace vqc {
 namespace detail {
+VQC_DEFINE_GRAD_CLASS(3, 0)
 VQC_DEFINE_METRIC_GRAD(3)
 }  // namespace detail
 }  // namespace vqc

@@ -19,7 +19,11 @@ void dispatch_pair_grad(int la, int lb, int lc, bool cart, const ThreeCenterGrad
     case 20: dispatch_3c_grad_20(lc, cart, args, nchunk, s); break;
     case 21: dispatch_3c_grad_21(lc, cart, args, nchunk, s); break;
     case 22: dispatch_3c_grad_22(lc, cart, args, nchunk, s); break;
-    default: throw NotImplemented("orbital angular momentum above d");
+    case 30: dispatch_3c_grad_30(lc, cart, args, nchunk, s); break;
+    case 31: dispatch_3c_grad_31(lc, cart, args, nchunk, s); break;
+    case 32: dispatch_3c_grad_32(lc, cart, args, nchunk, s); break;
+    case 33: dispatch_3c_grad_33(lc, cart, args, nchunk, s); break;
+    default: throw NotImplemented("orbital angular momentum above f");
   }
 }
 
@@ -41,7 +45,11 @@ void dispatch_1e_grad(int la, int lb, const OneElectronGradArgs& args, hipStream
     case 20: dispatch_1e_grad_20(args, s); break;
     case 21: dispatch_1e_grad_21(args, s); break;
     case 22: dispatch_1e_grad_22(args, s); break;
-    default: throw NotImplemented("orbital angular momentum above d");
+    case 30: dispatch_1e_grad_30(args, s); break;
+    case 31: dispatch_1e_grad_31(args, s); break;
+    case 32: dispatch_1e_grad_32(args, s); break;
+    case 33: dispatch_1e_grad_33(args, s); break;
+    default: throw NotImplemented("orbital angular momentum above f");
   }
 }
 
@@ -263,7 +271,7 @@ void compute_one_electron_gradient(const System& sys, const DeviceBasis& basis, 
     if (classes[key].empty()) continue;
     const int la = key / 10, lb = key % 10;
     const int ntask = static_cast<int>(classes[key].size());
-    const int sub = one_electron_lanes(natom);
+    const int sub = one_electron_lanes(natom, one_electron_pair_bytes(la, lb) / 3 * 2);  // D and W slots
     const int nblock = one_electron_grad_blocks(ntask, sub);
     std::vector<int2> ta(ntask);
     for (int t = 0; t < ntask; ++t)

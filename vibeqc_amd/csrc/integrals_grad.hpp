@@ -42,6 +42,7 @@ struct ThreeCenterGradArgs {
   double prim_cutoff;
   double* out_ab;  // [ntask][gridDim.y][6]: d/dA xyz, d/dB xyz summed over the block's 64 functions
   double* out_c;   // [gridDim.x][nfn][3]: d/dC of each function summed over the block's tasks
+  int accumulate;  // add into out_ab / out_c instead of writing them
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -50,40 +51,77 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Contracted Cartesian (a b| from the (e0| accumulator whose lowest degree is
-// LO: (x-B)^b = sum_k C(b,k) (x-A)^k (A-B)^(b-k).
-template <int LO, int NE>
-__device__ __forceinline__ double hrr_value(const double (&acc)[NE], int ax, int ay, int az, int bx, int by, int bz,
-                                            double ABx, double ABy, double ABz) {
+// Degree ranges of the (e0| accumulators of one orbital class: raised on A
+// (a + 1_d, b), raised on B (a, b + 1_d; none for the metric) and lowered
+// (a - 1_d, b and a, b - 1_d), with the entries each holds.
+template <int LA, int LB, bool METRIC>
+struct GradAccumulators {
+  static constexpr int LAB = LA + LB, LHI = LAB + 1;
+  static constexpr int UP_A = LA + 1, UP_B = LA;
+  static constexpr int DN_LO = LA > 0 ? LA - 1 : 0, DN_HI = LAB - 1;
+  static constexpr int NUP_A = ne_offset(UP_A, LHI + 1);
+  static constexpr int NUP_B = METRIC ? 0 : ne_offset(UP_B, LHI + 1);
+  static constexpr int NDN = DN_HI >= DN_LO ? ne_offset(DN_LO, DN_HI + 1) : 0;
+  // Entries held by a launch that keeps the accumulators selected by `mask`
+  // (1: raised on A, 2: raised on B, 4: lowered).
+  static constexpr int entries(int mask) {
+    return ((mask & 1) ? NUP_A : 0) + ((mask & 2) ? NUP_B : 0) + ((mask & 4) ? NDN : 0);
+  }
+};
+
+// Contracted Cartesian (a b| from one lane's accumulator column in LDS
+// (entries 64 doubles apart), whose lowest degree is LO:
+// (x-B)^b = sum_k C(b,k) (x-A)^k (A-B)^(b-k).
+template <int LO, int AX, int AY, int AZ, int BX, int BY, int BZ>
+__device__ __forceinline__ double hrr_column(const double* acc, double ABx, double ABy, double ABz) {
   double v = 0.0;
+  static_for<BX + 1>([&](auto KXI) {
+    constexpr int kx = VQC_CI(KXI);
+    static_for<BY + 1>([&](auto KYI) {
+      constexpr int ky = VQC_CI(KYI);
+      static_for<BZ + 1>([&](auto KZI) {
+        constexpr int kz = VQC_CI(KZI);
+        double w = binom(BX, kx) * binom(BY, ky) * binom(BZ, kz);
 #pragma unroll
-  for (int kx = 0; kx <= bx; ++kx)
+        for (int r = 0; r < BX - kx; ++r) w *= ABx;
 #pragma unroll
-    for (int ky = 0; ky <= by; ++ky)
+        for (int r = 0; r < BY - ky; ++r) w *= ABy;
 #pragma unroll
-      for (int kz = 0; kz <= bz; ++kz) {
-        double w = binom(bx, kx) * binom(by, ky) * binom(bz, kz);
-#pragma unroll
-        for (int r = 0; r < bx - kx; ++r) w *= ABx;
-#pragma unroll
-        for (int r = 0; r < by - ky; ++r) w *= ABy;
-#pragma unroll
-        for (int r = 0; r < bz - kz; ++r) w *= ABz;
-        const int le = ax + ay + az + kx + ky + kz;
-        v += w * acc[ne_offset(LO, le) + cart_index(ax + kx, ay + ky, az + kz)];
-      }
+        for (int r = 0; r < BZ - kz; ++r) w *= ABz;
+        constexpr int le = AX + AY + AZ + kx + ky + kz;
+        v += w * acc[(ne_offset(LO, le) + cart_index(AX + kx, AY + ky, AZ + kz)) * 64];
+      });
+    });
+  });
   return v;
 }
 
-template <int LA, int LB, int LC, bool METRIC, bool CART>
+// d/dA, d/dB and d/dC of (ab|P), contracted with the weights W, for the
+// shell pairs of one chunk (blockIdx.x) and 64 auxiliary functions (one per
+// lane).  Per bra primitive pair the auxiliary primitives are summed in
+// Hermite space (as in the value kernel), contracted with the bra's Hermite
+// coefficients once, and added into the (e0| accumulators, which live in LDS
+// (one column per lane, so no lane reads another's entries and no barrier is
+// needed).  The accumulators selected by MASK are kept; a class whose three
+// accumulators exceed the LDS budget runs as several launches with disjoint
+// masks that add into the same outputs.  Every register-array index is a
+// compile-time constant.
+template <int LA, int LB, int LC, bool METRIC, bool CART, int MASK>
 __global__ __launch_bounds__(64) void three_center_grad_kernel(const ThreeCenterGradArgs a) {
-  constexpr int LAB = LA + LB;
-  constexpr int LO = LA > 0 ? LA - 1 : 0;
-  constexpr int LHI = LAB + 1;
+  using G = GradAccumulators<LA, LB, METRIC>;
+  constexpr int LAB = G::LAB, LHI = G::LHI;
   constexpr int L = LHI + LC;
-  constexpr int NE = ne_offset(LO, LHI + 1);
   constexpr int NH = hermite_count(LC, CART);
   constexpr int NA = nc(LA), NB = nc(LB);
+  constexpr bool KEEP_A = (MASK & 1) != 0, KEEP_B = !METRIC && (MASK & 2) != 0, KEEP_D = (MASK & 4) != 0 && G::NDN > 0;
+  constexpr int OFF_A = 0;
+  constexpr int OFF_B = KEEP_A ? G::NUP_A : 0;
+  constexpr int OFF_D = OFF_B + (KEEP_B ? G::NUP_B : 0);
+  constexpr int NACC = OFF_D + (KEEP_D ? G::NDN : 0);
+  constexpr int LO = LA > 0 ? LA - 1 : 0;  // lowest (e0| degree any accumulator needs
+  extern __shared__ double grad_acc[];  // [NACC][64]
+  double* acc = grad_acc + threadIdx.x;
+
   const int f = blockIdx.y * 64 + threadIdx.x;
   const bool active = f < a.nfn;
   const AuxFunction fn = a.fns[active ? f : 0];
@@ -99,7 +137,56 @@ __global__ __launch_bounds__(64) void three_center_grad_kernel(const ThreeCenter
     const int2 task = a.tasks[task_id];
     const DevShell A = a.bra_shells[task.x];
     const DevShell B = METRIC ? A : a.bra_shells[task.y];
-    // Cartesian weights of this shell pair and auxiliary function.
+#pragma unroll 1
+    for (int k = 0; k < NACC; ++k) acc[k * 64] = 0.0;
+    const double ABx = A.x - B.x, ABy = A.y - B.y, ABz = A.z - B.z;
+    const double AB2 = ABx * ABx + ABy * ABy + ABz * ABz;
+    const int nprim_b = METRIC ? 1 : B.nprim;
+    for (int ia = 0; ia < A.nprim; ++ia) {
+      const double ea = a.bra_exps[A.prim_off + ia];
+      const double ca = a.bra_coefs[A.prim_off + ia];
+      for (int ib = 0; ib < nprim_b; ++ib) {
+        const double eb = METRIC ? 0.0 : a.bra_exps[B.prim_off + ib];
+        const double cb = METRIC ? 1.0 : a.bra_coefs[B.prim_off + ib];
+        const double p = ea + eb;
+        const double ip = 1.0 / p;
+        const double kab = ca * cb * exp(-ea * eb * ip * AB2);
+        if (fabs(kab) < a.prim_cutoff) continue;
+        const double Px = (ea * A.x + eb * B.x) * ip;
+        const double Py = (ea * A.y + eb * B.y) * ip;
+        const double Pz = (ea * A.z + eb * B.z) * ip;
+        double ws[ntuv(LHI)];
+#pragma unroll
+        for (int k = 0; k < ntuv(LHI); ++k) ws[k] = 0.0;
+        for (int ic = 0; ic < C.nprim; ++ic) {
+          const double g = a.aux_exps[C.prim_off + ic];
+          const double cg = a.aux_coefs[C.prim_off + ic];
+          const double q = p + g;
+          const double alpha = p * g / q;
+          const double X = Px - C.x, Y = Py - C.y, Z = Pz - C.z;
+          double F[L + 1];
+          boys_eval<L>(alpha * (X * X + Y * Y + Z * Z), a.boys, F);
+          double R[ntuv(L)];
+          hermite_r<L>(alpha, X, Y, Z, F, R);
+          const double pref = 2.0 * 17.493418327624862 /* pi^{5/2} */ / (p * g * sqrt(q)) * cg;
+          ket_contract<LHI, LC, CART, NH, L>(hc, 0.5 / g, pref, R, ws);
+        }
+        double Ex[LHI + 1][LHI + 1], Ey[LHI + 1][LHI + 1], Ez[LHI + 1][LHI + 1];
+        hermite_e0<LHI>(Px - A.x, 0.5 * ip, Ex);
+        hermite_e0<LHI>(Py - A.y, 0.5 * ip, Ey);
+        hermite_e0<LHI>(Pz - A.z, 0.5 * ip, Ez);
+        const double pa = 2.0 * ea * kab, pb = 2.0 * eb * kab;
+        bra_contract<LO, LHI, LHI>(Ex, Ey, Ez, ws, [&](auto LI, auto CI, double v) {
+          constexpr int le = VQC_CI(LI), c = VQC_CI(CI);
+          if constexpr (KEEP_A && le >= G::UP_A) acc[(OFF_A + ne_offset(G::UP_A, le) + c) * 64] += pa * v;
+          if constexpr (KEEP_B && le >= G::UP_B) acc[(OFF_B + ne_offset(G::UP_B, le) + c) * 64] += pb * v;
+          if constexpr (KEEP_D && le >= G::DN_LO && le <= G::DN_HI)
+            acc[(OFF_D + ne_offset(G::DN_LO, le) + c) * 64] += kab * v;
+        });
+      }
+    }
+
+    // Cartesian weights w_c = T_a^T W T_b of this shell pair and function.
     double wc[NA][NB];
     {
       double wp[NA][NB];
@@ -132,125 +219,33 @@ __global__ __launch_bounds__(64) void three_center_grad_kernel(const ThreeCenter
         }
     }
 
-    double acc0[NE], accA[NE], accB[NE];
-#pragma unroll
-    for (int k = 0; k < NE; ++k) acc0[k] = accA[k] = accB[k] = 0.0;
-    const double ABx = A.x - B.x, ABy = A.y - B.y, ABz = A.z - B.z;
-    const double AB2 = ABx * ABx + ABy * ABy + ABz * ABz;
-    const int nprim_b = METRIC ? 1 : B.nprim;
-    for (int ia = 0; ia < A.nprim; ++ia) {
-      const double ea = a.bra_exps[A.prim_off + ia];
-      const double ca = a.bra_coefs[A.prim_off + ia];
-      for (int ib = 0; ib < nprim_b; ++ib) {
-        const double eb = METRIC ? 0.0 : a.bra_exps[B.prim_off + ib];
-        const double cb = METRIC ? 1.0 : a.bra_coefs[B.prim_off + ib];
-        const double p = ea + eb;
-        const double ip = 1.0 / p;
-        const double kab = ca * cb * exp(-ea * eb * ip * AB2);
-        if (fabs(kab) < a.prim_cutoff) continue;
-        const double Px = (ea * A.x + eb * B.x) * ip;
-        const double Py = (ea * A.y + eb * B.y) * ip;
-        const double Pz = (ea * A.z + eb * B.z) * ip;
-        double Ex[LHI + 1][LHI + 1], Ey[LHI + 1][LHI + 1], Ez[LHI + 1][LHI + 1];
-        hermite_e0<LHI>(Px - A.x, 0.5 * ip, Ex);
-        hermite_e0<LHI>(Py - A.y, 0.5 * ip, Ey);
-        hermite_e0<LHI>(Pz - A.z, 0.5 * ip, Ez);
-        for (int ic = 0; ic < C.nprim; ++ic) {
-          const double g = a.aux_exps[C.prim_off + ic];
-          const double cg = a.aux_coefs[C.prim_off + ic];
-          const double q = p + g;
-          const double alpha = p * g / q;
-          const double X = Px - C.x, Y = Py - C.y, Z = Pz - C.z;
-          double F[L + 1];
-          boys_eval<L>(alpha * (X * X + Y * Y + Z * Z), a.boys, F);
-          double R[ntuv(L)];
-          hermite_r<L>(alpha, X, Y, Z, F, R);
-          const double i2g = 0.5 / g;
-          double scale[LC + 1];
-          scale[0] = 1.0;
-#pragma unroll
-          for (int k = 1; k <= LC; ++k) scale[k] = scale[k - 1] * i2g;
-          double W[ntuv(LHI)];
-#pragma unroll
-          for (int s = 0; s <= LHI; ++s)
-#pragma unroll
-            for (int t = 0; t <= s; ++t)
-#pragma unroll
-              for (int u = 0; u <= s - t; ++u) {
-                const int v = s - t - u;
-                double w = 0.0;
-                int k = 0;
-#pragma unroll
-                for (int d = LC; d >= 0; d -= 2) {
-#pragma unroll
-                  for (int tt = 0; tt <= d; ++tt)
-#pragma unroll
-                    for (int uu = 0; uu <= d - tt; ++uu) {
-                      const int vv = d - tt - uu;
-                      if (k < NH) w += hc[k] * scale[(LC + d) / 2] * R[tuv(t + tt, u + uu, v + vv)];
-                      ++k;
-                    }
-                  if (!CART) break;
-                }
-                W[tuv(t, u, v)] = w;
-              }
-          const double pref = 2.0 * 17.493418327624862 /* pi^{5/2} */ / (p * g * sqrt(q)) * kab * cg;
-          const double pa = 2.0 * ea * pref, pb = 2.0 * eb * pref;
-#pragma unroll
-          for (int le = LO; le <= LHI; ++le)
-#pragma unroll
-            for (int c = 0; c < nc(le); ++c) {
-              const int ex = cart_lx(le, c), ey = cart_ly(le, c), ez = cart_lz(le, c);
-              double s = 0.0;
-#pragma unroll
-              for (int t = 0; t <= ex; ++t) {
-                double su = 0.0;
-#pragma unroll
-                for (int u = 0; u <= ey; ++u) {
-                  double sv = 0.0;
-#pragma unroll
-                  for (int v = 0; v <= ez; ++v) sv += Ez[ez][v] * W[tuv(t, u, v)];
-                  su += Ey[ey][u] * sv;
-                }
-                s += Ex[ex][t] * su;
-              }
-              const int k = ne_offset(LO, le) + c;
-              if (le <= LAB) acc0[k] += pref * s;
-              if (le >= LA + 1) accA[k] += pa * s;
-              if (!METRIC && le >= LA) accB[k] += pb * s;
-            }
-        }
-      }
-    }
-
     double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-      for (int ib = 0; ib < NB; ++ib) {
-        const int ax[3] = {cart_lx(LA, ia), cart_ly(LA, ia), cart_lz(LA, ia)};
-        const int bx[3] = {cart_lx(LB, ib), cart_ly(LB, ib), cart_lz(LB, ib)};
+    static_for<NA>([&](auto IA) {
+      constexpr int ia = VQC_CI(IA);
+      constexpr int ax = cart_lx(LA, ia), ay = cart_ly(LA, ia), az = cart_lz(LA, ia);
+      static_for<NB>([&](auto IB) {
+        constexpr int ib = VQC_CI(IB);
+        constexpr int bx = cart_lx(LB, ib), by = cart_ly(LB, ib), bz = cart_lz(LB, ib);
         const double w = wc[ia][ib];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const int up[3] = {ax[0] + (d == 0), ax[1] + (d == 1), ax[2] + (d == 2)};
-          double v = hrr_value<LO, NE>(accA, up[0], up[1], up[2], bx[0], bx[1], bx[2], ABx, ABy, ABz);
-          if (ax[d] > 0) {
-            const int dn[3] = {ax[0] - (d == 0), ax[1] - (d == 1), ax[2] - (d == 2)};
-            v -= ax[d] * hrr_value<LO, NE>(acc0, dn[0], dn[1], dn[2], bx[0], bx[1], bx[2], ABx, ABy, ABz);
-          }
+        static_for<3>([&](auto DI) {
+          constexpr int d = VQC_CI(DI);
+          constexpr int dx = d == 0, dy = d == 1, dz = d == 2;
+          constexpr int a_d = d == 0 ? ax : (d == 1 ? ay : az);
+          constexpr int b_d = d == 0 ? bx : (d == 1 ? by : bz);
+          double v = 0.0, vb = 0.0;
+          if constexpr (KEEP_A)
+            v += hrr_column<G::UP_A, ax + dx, ay + dy, az + dz, bx, by, bz>(acc + OFF_A * 64, ABx, ABy, ABz);
+          if constexpr (KEEP_D && a_d > 0)
+            v -= a_d * hrr_column<G::DN_LO, ax - dx, ay - dy, az - dz, bx, by, bz>(acc + OFF_D * 64, ABx, ABy, ABz);
+          if constexpr (KEEP_B)
+            vb += hrr_column<G::UP_B, ax, ay, az, bx + dx, by + dy, bz + dz>(acc + OFF_B * 64, ABx, ABy, ABz);
+          if constexpr (!METRIC && KEEP_D && b_d > 0)
+            vb -= b_d * hrr_column<G::DN_LO, ax, ay, az, bx - dx, by - dy, bz - dz>(acc + OFF_D * 64, ABx, ABy, ABz);
           gA[d] += w * v;
-          if (!METRIC) {
-            const int bu[3] = {bx[0] + (d == 0), bx[1] + (d == 1), bx[2] + (d == 2)};
-            double vb = hrr_value<LO, NE>(accB, ax[0], ax[1], ax[2], bu[0], bu[1], bu[2], ABx, ABy, ABz);
-            if (bx[d] > 0) {
-              const int bd[3] = {bx[0] - (d == 0), bx[1] - (d == 1), bx[2] - (d == 2)};
-              vb -= bx[d] * hrr_value<LO, NE>(acc0, ax[0], ax[1], ax[2], bd[0], bd[1], bd[2], ABx, ABy, ABz);
-            }
-            gB[d] += w * vb;
-          }
-        }
-      }
+          gB[d] += w * vb;
+        });
+      });
+    });
 #pragma unroll
     for (int d = 0; d < 3; ++d) gc[d] -= gA[d] + gB[d];
     double* o = a.out_ab + (static_cast<std::size_t>(task_id) * gridDim.y + blockIdx.y) * 6;
@@ -259,15 +254,15 @@ __global__ __launch_bounds__(64) void three_center_grad_kernel(const ThreeCenter
       const double sa = wave_sum(gA[d]);
       const double sb = wave_sum(gB[d]);
       if (threadIdx.x == 0) {
-        o[d] = sa;
-        o[3 + d] = sb;
+        o[d] = a.accumulate ? o[d] + sa : sa;
+        o[3 + d] = a.accumulate ? o[3 + d] + sb : sb;
       }
     }
   }
   if (active) {
     double* o = a.out_c + (static_cast<std::size_t>(blockIdx.x) * a.nfn + f) * 3;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) o[d] = gc[d];
+    for (int d = 0; d < 3; ++d) o[d] = a.accumulate ? o[d] + gc[d] : gc[d];
   }
 }
 
@@ -484,12 +479,37 @@ __global__ __launch_bounds__(64) void one_electron_grad_kernel(const OneElectron
   }
 }
 
+// LDS budget of one gradient block's accumulators: a gfx950 workgroup may
+// hold the CU's whole 160 KiB (the f-f class needs 134 KiB, d-d 60 KiB).
+constexpr int kGradAccumulatorBytes = 160 << 10;
+
+template <int LA, int LB, int LC, bool METRIC, bool CART, int MASK>
+void launch_3c_grad_pass(ThreeCenterGradArgs args, bool accumulate, int nchunk, hipStream_t s) {
+  constexpr std::size_t lds = std::size_t{64} * sizeof(double) * GradAccumulators<LA, LB, METRIC>::entries(MASK);
+  static_assert(lds <= kGradAccumulatorBytes, "gradient accumulators exceed the LDS budget");
+  args.accumulate = accumulate ? 1 : 0;
+  dim3 grid(nchunk, (args.nfn + 63) / 64);
+  hipLaunchKernelGGL((three_center_grad_kernel<LA, LB, LC, METRIC, CART, MASK>), grid, dim3(64), lds, s, args);
+  VQC_LAUNCH("three_center_grad_kernel");
+}
+
+// One launch when all three accumulators fit the LDS budget, else two
+// (raised-on-A with lowered, then raised-on-B) or three.
 template <int LA, int LB, int LC, bool METRIC, bool CART>
 void launch_3c_grad(const ThreeCenterGradArgs& args, int nchunk, hipStream_t s) {
   if (args.ntask == 0 || args.nfn == 0) return;
-  dim3 grid(nchunk, (args.nfn + 63) / 64);
-  hipLaunchKernelGGL((three_center_grad_kernel<LA, LB, LC, METRIC, CART>), grid, dim3(64), 0, s, args);
-  VQC_LAUNCH("three_center_grad_kernel");
+  using G = GradAccumulators<LA, LB, METRIC>;
+  constexpr int cap = kGradAccumulatorBytes / (64 * static_cast<int>(sizeof(double)));
+  if constexpr (G::entries(7) <= cap) {
+    launch_3c_grad_pass<LA, LB, LC, METRIC, CART, 7>(args, false, nchunk, s);
+  } else if constexpr (G::entries(5) <= cap && G::entries(2) <= cap) {
+    launch_3c_grad_pass<LA, LB, LC, METRIC, CART, 5>(args, false, nchunk, s);
+    launch_3c_grad_pass<LA, LB, LC, METRIC, CART, 2>(args, true, nchunk, s);
+  } else {
+    launch_3c_grad_pass<LA, LB, LC, METRIC, CART, 1>(args, false, nchunk, s);
+    launch_3c_grad_pass<LA, LB, LC, METRIC, CART, 2>(args, true, nchunk, s);
+    launch_3c_grad_pass<LA, LB, LC, METRIC, CART, 4>(args, true, nchunk, s);
+  }
 }
 
 template <int LA, int LB, bool METRIC>
@@ -543,6 +563,10 @@ VQC_DECLARE_GRAD_CLASS(1, 1)
 VQC_DECLARE_GRAD_CLASS(2, 0)
 VQC_DECLARE_GRAD_CLASS(2, 1)
 VQC_DECLARE_GRAD_CLASS(2, 2)
+VQC_DECLARE_GRAD_CLASS(3, 0)
+VQC_DECLARE_GRAD_CLASS(3, 1)
+VQC_DECLARE_GRAD_CLASS(3, 2)
+VQC_DECLARE_GRAD_CLASS(3, 3)
 void dispatch_metric_grad_0(int lc, bool cart, const ThreeCenterGradArgs& a, int nchunk, hipStream_t s);
 void dispatch_metric_grad_1(int lc, bool cart, const ThreeCenterGradArgs& a, int nchunk, hipStream_t s);
 void dispatch_metric_grad_2(int lc, bool cart, const ThreeCenterGradArgs& a, int nchunk, hipStream_t s);

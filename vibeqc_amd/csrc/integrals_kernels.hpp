@@ -432,12 +432,19 @@ struct OneElectronArgs {
   int sub;  // lanes per shell pair (1, 2, 4 or 8): they split the nuclei of V
 };
 
-// Lanes per shell pair for the one-electron kernel: enough that each lane
-// takes about four nuclei of the V sum, at most eight.
-inline int one_electron_lanes(int natom) {
+// Lanes per shell pair for the one-electron kernels: enough that each lane
+// takes about four nuclei of the V sum, at most eight, and at least enough
+// that the 64-lane block's LDS slots (`pair_bytes` per shell pair, 64 / sub
+// pairs per block) stay within 64 KiB.
+constexpr std::size_t kOneElectronLds = std::size_t{64} << 10;
+inline int one_electron_lanes(int natom, std::size_t pair_bytes) {
   int sub = 1;
-  while (sub < 8 && sub * 4 < natom) sub *= 2;
+  while (sub < 8 && (sub * 4 < natom || (64 / sub) * pair_bytes > kOneElectronLds)) sub *= 2;
   return sub;
+}
+// LDS bytes per shell pair of the value kernel (S, T, V slots).
+inline std::size_t one_electron_pair_bytes(int la, int lb) {
+  return std::size_t{3} * ((la + 1) * (la + 2) / 2) * ((lb + 1) * (lb + 2) / 2) * sizeof(double);
 }
 
 // Overlap factors E^{ij}_0 along one axis for i <= I, j <= J by the
@@ -688,6 +695,10 @@ VQC_DECLARE_CLASS(1, 1)
 VQC_DECLARE_CLASS(2, 0)
 VQC_DECLARE_CLASS(2, 1)
 VQC_DECLARE_CLASS(2, 2)
+VQC_DECLARE_CLASS(3, 0)
+VQC_DECLARE_CLASS(3, 1)
+VQC_DECLARE_CLASS(3, 2)
+VQC_DECLARE_CLASS(3, 3)
 void dispatch_metric_0(int lc, bool cart, const ThreeCenterArgs& a, int ntask, hipStream_t s);
 void dispatch_metric_1(int lc, bool cart, const ThreeCenterArgs& a, int ntask, hipStream_t s);
 void dispatch_metric_2(int lc, bool cart, const ThreeCenterArgs& a, int ntask, hipStream_t s);
