@@ -216,6 +216,18 @@ GENERATIVEQC_API generativeqc_status vibeqc_fock_plan_one_electron(const generat
 GENERATIVEQC_API generativeqc_status vibeqc_fock_plan_shape(const generativeqc_fock_plan* plan,
                                                             uint64_t* shape6);
 
+/* Extension (no counterpart in the reference): how the plan honours its
+ * device_budget_bytes control.  [0] 1 when the fitted tensor does not fit the
+ * budget and is regenerated in auxiliary blocks for every Fock build (the
+ * role of the reference's budgeted tiles, density_fitting.cpp:1041-1218),
+ * 0 when it is resident; [1] auxiliary functions per block; [2] three-centre
+ * row tiles per block.  A streamed plan builds J/K and runs its SCF in FP64;
+ * nuclear derivatives and the allocation-free entry need a resident plan
+ * (NOT_IMPLEMENTED).  A budget below the metric, its factor and one block
+ * fails plan creation with OUT_OF_MEMORY. */
+GENERATIVEQC_API generativeqc_status vibeqc_fock_plan_streaming(const generativeqc_fock_plan* plan,
+                                                                uint64_t* streaming3);
+
 /* Extension (no counterpart in the reference): statistics of the plan's last
  * generativeqc_fock_plan_solve: [0] SCF iterations, [1] of them with
  * reduced-precision Fock builds, [2] new densities obtained by purification

@@ -318,6 +318,7 @@ generativeqc_status generativeqc_fock_plan_create(generativeqc_context* context,
       throw std::invalid_argument("a Fock plan requires at least one J or K term");
     vqc::PlanOptions opts;
     opts.screening = screening;
+    opts.device_budget = controls ? controls->device_budget_bytes : 0;
     opts.metric_threshold = requested == 0.0 ? 1e-10 : requested;
     opts.coulomb = spec->coulomb.present != 0;
     opts.exchange = spec->exchange.present != 0;
@@ -646,6 +647,15 @@ generativeqc_status vibeqc_fock_plan_shape(const generativeqc_fock_plan* plan, u
   return GENERATIVEQC_STATUS_SUCCESS;
 }
 
+generativeqc_status vibeqc_fock_plan_streaming(const generativeqc_fock_plan* plan, uint64_t* out3) {
+  if (!plan || !out3) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  const auto& p = *plan->plan;
+  out3[0] = p.streamed() ? 1 : 0;
+  out3[1] = static_cast<uint64_t>(p.auxiliary_block());
+  out3[2] = static_cast<uint64_t>(p.row_tiles());
+  return GENERATIVEQC_STATUS_SUCCESS;
+}
+
 generativeqc_status vibeqc_fock_plan_last_solve_stats(const generativeqc_fock_plan* plan, uint64_t* stats3) {
   if (!plan || !stats3) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
   std::copy(plan->solve_stats, plan->solve_stats + 3, stats3);
@@ -845,7 +855,9 @@ generativeqc_status generativeqc_calculation_prepare(generativeqc_context* conte
     const vqc::System aux = descriptor->density_fitting_auxiliary_basis
                                 ? auxiliary_from_template(descriptor->density_fitting_auxiliary_basis->data, system->data)
                                 : system->data;
-    out->plan = std::make_unique<vqc::DfPlan>(context->device, system->data, aux, opts.plan);
+    vqc::PlanOptions plan_options = opts.plan;
+    plan_options.device_budget = out->budget;
+    out->plan = std::make_unique<vqc::DfPlan>(context->device, system->data, aux, plan_options);
     out->plan->set_precision(opts.precision);
     *calculation = out.release();
     return GENERATIVEQC_STATUS_SUCCESS;

@@ -81,11 +81,12 @@ long double boys_series(int m, long double t) {
   return std::exp(-t) * sum;
 }
 
-// Per-pair task lists grouped by the (larger l, smaller l) class.
-std::vector<std::vector<int2>> pair_tasks(const System& sys) {
+// Per-pair task lists grouped by the (larger l, smaller l) class, for the
+// shell pairs (i, j <= i) with i in [first, last).
+std::vector<std::vector<int2>> pair_tasks(const System& sys, int first = 0, int last = -1) {
   std::vector<std::vector<int2>> classes(100);
-  const int n = static_cast<int>(sys.shells.size());
-  for (int i = 0; i < n; ++i)
+  const int n = last < 0 ? static_cast<int>(sys.shells.size()) : last;
+  for (int i = first; i < n; ++i)
     for (int j = 0; j <= i; ++j) {
       const int li = sys.shells[i].l, lj = sys.shells[j].l;
       const int2 t = li >= lj ? make_int2(i, j) : make_int2(j, i);
@@ -287,12 +288,16 @@ void compute_metric(const AuxiliaryLayout& aux, const double* boys, double* M, h
 }
 
 void compute_three_center(const System& orb, const DeviceBasis& basis, const AuxiliaryLayout& aux,
-                          const double* boys, double* A, std::size_t ld, double screening, hipStream_t s) {
+                          const double* boys, double* A, std::size_t ld, double screening, hipStream_t s,
+                          int shell_begin, int shell_end) {
   const auto& b = aux.basis;
   const bool cart = b.representation == GENERATIVEQC_BASIS_CARTESIAN;
-  const std::size_t npair = pair_count(basis.nfunctions);
-  VQC_HIP(hipMemsetAsync(A, 0, npair * ld * sizeof(double), s));
-  const auto classes = pair_tasks(orb);
+  if (shell_end < 0) shell_end = basis.nshell;
+  const std::size_t f0 = shell_begin < basis.nshell ? basis.host_shells[shell_begin].fn_off : basis.nfunctions;
+  const std::size_t f1 = shell_end < basis.nshell ? basis.host_shells[shell_end].fn_off : basis.nfunctions;
+  const std::size_t row0 = pair_count(f0);
+  VQC_HIP(hipMemsetAsync(A, 0, (pair_count(f1) - row0) * ld * sizeof(double), s));
+  const auto classes = pair_tasks(orb, shell_begin, shell_end);
   std::vector<DeviceArray<int2>> keep;
   for (int key = 0; key < 100; ++key) {
     if (classes[key].empty()) continue;
@@ -321,6 +326,7 @@ void compute_three_center(const System& orb, const DeviceBasis& basis, const Aux
       args.out = A;
       args.ld = ld;
       args.prim_cutoff = screening;
+      args.row0 = row0;
       dispatch_pair(la, lb, lc, cart && lc >= 2, args, static_cast<int>(classes[key].size()), s);
     }
   }

@@ -63,8 +63,12 @@ void compute_metric(const AuxiliaryLayout& aux, const double* boys, double* M, h
 
 // Packed (mu nu|P): row p = mu(mu+1)/2 + nu (mu >= nu), row stride ld >= naux
 // (internal auxiliary order).  The npair x ld buffer is fully overwritten
-// (screened entries and padding columns are zero).
+// (screened entries and padding columns are zero).  With a shell range, only
+// the rows of the pairs whose larger shell index lies in [shell_begin,
+// shell_end) are computed: rows f0(f0+1)/2 up to f1(f1+1)/2, f0 and f1 the
+// first AO functions of shell_begin and shell_end, stored from A's first row.
 void compute_three_center(const System& orb, const DeviceBasis& basis, const AuxiliaryLayout& aux,
-                          const double* boys, double* A, std::size_t ld, double screening, hipStream_t s);
+                          const double* boys, double* A, std::size_t ld, double screening, hipStream_t s,
+                          int shell_begin = 0, int shell_end = -1);
 
 }  // namespace vqc

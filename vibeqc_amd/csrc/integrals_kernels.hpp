@@ -210,6 +210,7 @@ struct ThreeCenterArgs {
   double* out;
   std::size_t ld;
   double prim_cutoff;
+  std::size_t row0;  // packed pair row stored at out[0] (orbital pairs only)
 };
 
 // Ket contraction of one auxiliary primitive into Hermite bra space:
@@ -407,7 +408,7 @@ __global__ __launch_bounds__(64) void three_center_kernel(const ThreeCenterArgs 
         const int mu = A.fn_off + ma, nu = B.fn_off + mb;
         if (task.x == task.y && nu > mu) continue;
         const std::size_t hi = mu > nu ? mu : nu, lo = mu > nu ? nu : mu;
-        a.out[(hi * (hi + 1) / 2 + lo) * a.ld + fn.index] = v;
+        a.out[(hi * (hi + 1) / 2 + lo - a.row0) * a.ld + fn.index] = v;
       }
     }
   }

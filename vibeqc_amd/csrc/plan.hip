@@ -70,6 +70,21 @@ class Trace {
   std::chrono::steady_clock::time_point start_, last_;
 };
 
+// Zero the strict upper triangle of a column-major n x n matrix.
+__global__ void zero_strict_upper_kernel(double* a, int n) {
+  const std::size_t total = static_cast<std::size_t>(n) * n;
+  for (std::size_t k = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; k < total;
+       k += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    const std::size_t row = k % n, col = k / n;
+    if (row < col) a[k] = 0.0;
+  }
+}
+
+// n x n doubles held besides the fitted tensor by a plan and its SCF: S, H,
+// J, K, D, the Fock matrix, residual temporaries and an eight-deep DIIS
+// history of Fock and error matrices.
+constexpr std::size_t kSquareMatrices = 32;
+
 void check_basis_limits(const System& sys, int lmax, const char* what) {
   for (const auto& s : sys.shells)
     if (static_cast<int>(s.l) > lmax)
@@ -119,7 +134,22 @@ void DfPlan::sync() { VQC_HIP(hipStreamSynchronize(stream_)); }
 std::size_t DfPlan::device_bytes() const {
   return boys_.bytes() + S_.bytes() + H_.bytes() + B64_.bytes() + B32_.bytes() + dpacked_.bytes() +
          rho_.bytes() + rho_partial_.bytes() + J_.bytes() + K_.bytes() + syrk_partial_.bytes() + D_.bytes() +
-         c64_.bytes() + c32_.bytes() + ct_hi_.bytes() + ct_lo_.bytes() + U_.bytes();
+         c64_.bytes() + c32_.bytes() + ct_hi_.bytes() + ct_lo_.bytes() + U_.bytes() + L_.bytes() + G_.bytes() +
+         Ablk_.bytes() + Bblk_.bytes() + Jtmp_.bytes() + Ktmp_.bytes() + Ublk_.bytes();
+}
+
+std::size_t DfPlan::resident_bytes() const {
+  const std::size_t nn = static_cast<std::size_t>(n_) * n_, mm = static_cast<std::size_t>(naux_) * naux_;
+  const std::size_t doubles = npair_ * ldq_ + 2 * mm + static_cast<std::size_t>(n_) * std::max(nocc_, 1) * ldq_ +
+                              std::max(coulomb_partial_size(npair_, ldq_), static_cast<std::size_t>(n_) * ldq_) +
+                              exchange_partial_size(n_) + kSquareMatrices * nn;
+  return doubles * sizeof(double);
+}
+
+void DfPlan::require_resident(const char* what) const {
+  if (streamed_)
+    throw NotImplemented(std::string(what) +
+                         " needs the resident fitted tensor; this plan regenerates it within its device budget");
 }
 
 void DfPlan::setup(const System& aux) {
@@ -183,12 +213,24 @@ void DfPlan::setup(const System& aux) {
                                    : 0.0;
   const double sum_m2 = checksums ? sum_squares(M.data(), M.size()) : 0.0;
   trace.mark("metric");
+  streamed_ = opts_.device_budget > 0 && resident_bytes() > opts_.device_budget;
+  if (streamed_) {
+    // Keep only the metric factor; B is regenerated per Fock build.
+    if (2 * M.bytes() + kSquareMatrices * nn * sizeof(double) > opts_.device_budget)
+      throw OutOfMemory("DF memory budget cannot hold the metric and one contraction tile");
+    factor_metric(M, nullptr);
+    M.release();
+    plan_streaming();
+    trace.mark(cholesky_ ? "cholesky_inverse" : "eigen_cutoff");
+    trace.finish(n_, naux_, npair_);
+    return;
+  }
   B64_.resize(npair_ * ldq_);
   compute_three_center(orb_, ob, al, boys_.data(), B64_.data(), ldq_, opts_.screening * 1e-10, stream_);
   sync();
   const double sum_a = checksums ? sum_squares(B64_.data(), B64_.size()) : 0.0;
   trace.mark("three_center");
-  factor_metric(M, B64_);
+  factor_metric(M, B64_.data());
   trace.mark(cholesky_ ? "cholesky_trsm" : "eigen_cutoff_gemm");
   VQC_HIP(hipMemsetAsync(flag_.data(), 0, sizeof(int), stream_));
   check_finite(B64_.data(), B64_.size(), flag_.data(), stream_);
@@ -207,7 +249,7 @@ void DfPlan::setup(const System& aux) {
                  sum_m2);
 }
 
-void DfPlan::factor_metric(DeviceArray<double>& M, DeviceArray<double>& A) {
+void DfPlan::factor_metric(DeviceArray<double>& M, double* A) {
   const int na = naux_;
   const std::size_t mm = static_cast<std::size_t>(na) * na;
   std::vector<double> mdiag(na), ldiag(na);
@@ -236,10 +278,22 @@ void DfPlan::factor_metric(DeviceArray<double>& M, DeviceArray<double>& A) {
     cholesky_ = true;
     rank_ = na;
     condition_ = maxpiv / minpiv;
+    if (!A) {
+      // Streamed: G = L^-1 (lower), so B^T = G A^T block by block.
+      hipLaunchKernelGGL(zero_strict_upper_kernel, dim3(1024), dim3(256), 0, stream_, L.data(), na);
+      VQC_LAUNCH("zero_strict_upper_kernel");
+      VQC_BLAS(rocsolver_dtrtri(blas_, rocblas_fill_lower, rocblas_diagonal_non_unit, na, L.data(), na, info.data()));
+      info.download(&hinfo, 1, stream_);
+      sync();
+      if (hinfo != 0) throw NumericalFailure("metric Cholesky factor is singular");
+      G_ = std::move(L);
+      g_lower_ = true;
+      return;
+    }
     for (std::size_t c0 = 0; c0 < npair_; c0 += kTrsmColumns) {
       const int cols = static_cast<int>(std::min(kTrsmColumns, npair_ - c0));
       VQC_BLAS(rocblas_dtrsm(blas_, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
-                             rocblas_diagonal_non_unit, na, cols, &one, L.data(), na, A.data() + c0 * ldq_, ldq_));
+                             rocblas_diagonal_non_unit, na, cols, &one, L.data(), na, A + c0 * ldq_, ldq_));
     }
     sync();
     L_ = std::move(L);  // kept for the gradient's c = L^-T rho and Y = L^-T T
@@ -278,24 +332,31 @@ void DfPlan::factor_metric(DeviceArray<double>& M, DeviceArray<double>& A) {
   scale_columns_inv_sqrt(V.data(), winv.data(), na, Vs.data(), stream_);
   VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_transpose, na, na, na, &one, Vs.data(), na,
                          V.data(), na, &zero, X.data(), na));
+  if (!A) {
+    sync();
+    G_ = std::move(X);
+    g_lower_ = false;
+    return;
+  }
   DeviceArray<double> tmp(kTrsmColumns * ldq_);
   VQC_HIP(hipMemsetAsync(tmp.data(), 0, tmp.bytes(), stream_));
   for (std::size_t c0 = 0; c0 < npair_; c0 += kTrsmColumns) {
     const int cols = static_cast<int>(std::min(kTrsmColumns, npair_ - c0));
     VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, na, cols, na, &one, X.data(), na,
-                           A.data() + c0 * ldq_, ldq_, &zero, tmp.data(), ldq_));
-    copy_device(A.data() + c0 * ldq_, tmp.data(), static_cast<std::size_t>(cols) * ldq_, stream_);
+                           A + c0 * ldq_, ldq_, &zero, tmp.data(), ldq_));
+    copy_device(A + c0 * ldq_, tmp.data(), static_cast<std::size_t>(cols) * ldq_, stream_);
   }
   sync();
 }
 
 void DfPlan::set_precision(Precision p) {
-  prec_ = p;
+  // A streamed plan regenerates B in FP64 for every build and has no reduced copy.
+  prec_ = streamed_ ? Precision::FP64 : p;
   ensure_reduced();
 }
 
 void DfPlan::ensure_reduced() {
-  if (prec_ == Precision::FP64 || B32_.size()) return;
+  if (prec_ == Precision::FP64 || B32_.size() || streamed_) return;
   B32_.resize(npair_ * ldq_);
   convert_f64_to_f32(B64_.data(), B32_.data(), B32_.size(), stream_);
   sync();
@@ -310,7 +371,8 @@ void DfPlan::ensure_factor(int r) {
     dpacked_.resize(npair_);
     rho_.resize(ldq_);
     rho_partial_.resize(std::max(coulomb_partial_size(npair_, ldq_), static_cast<std::size_t>(n_) * ldq_));
-    syrk_partial_.resize(exchange_partial_size(n_));
+    // Streamed plans bound the split-K slices (partial_capacity) to ten n x n matrices.
+    syrk_partial_.resize(streamed_ ? std::min(exchange_partial_size(n_), 10 * nn) : exchange_partial_size(n_));
   }
   r = std::max(r, 1);
   const int ni = occupied_tile(r);
@@ -325,7 +387,7 @@ void DfPlan::ensure_factor(int r) {
   ct_hi_.resize(static_cast<std::size_t>(ldc_) * ldct);
   ct_lo_.resize(static_cast<std::size_t>(ldc_) * ldct);
   U_.release();
-  U_.resize(static_cast<std::size_t>(n_) * r * ldq_ * sizeof(double));
+  if (!streamed_) U_.resize(static_cast<std::size_t>(n_) * r * ldq_ * sizeof(double));  // streamed: Ublk_ per block
   ops_ = OccupiedOperands{};
   ops_.n = n_;
   ops_.ldc = ldc_;
@@ -338,6 +400,10 @@ void DfPlan::ensure_factor(int r) {
 }
 
 void DfPlan::build_jk(Precision p, const double* D, int r, int r_pos) {
+  if (streamed_) {
+    build_jk_streamed(D, r, r_pos, opts_.exchange);
+    return;
+  }
   if (p != Precision::FP64) ensure_reduced();
   const void* B = p == Precision::FP64 ? static_cast<const void*>(B64_.data()) : static_cast<const void*>(B32_.data());
   // When the U pass runs it also produces the rho partials, so J needs only
@@ -360,6 +426,155 @@ void DfPlan::build_jk(Precision p, const double* D, int r, int r_pos) {
       coulomb_rho(p, B, dpacked_.data(), npair_, ldq_, rho_.data(), rho_partial_.data(), stream_);
     }
     coulomb_matrix(p, B, rho_.data(), npair_, ldq_, n_, J_.data(), stream_);
+  }
+}
+
+// Streamed plans (the reference's budgeted DF planner,
+// src/scf/density_fitting.cpp:1041-1218, regenerates B in tiles when the full
+// tensor does not fit).  Here the fitted tensor is produced one auxiliary block
+// B[:, q0:q1] at a time: the three-centre rows are regenerated tile by tile
+// (shell ranges) and mapped by one GEMM, B^T[q0:q1, rows] = G[q0:q1, :] A^T.
+// J and K are sums over Q, so every block adds its own rho_Q, J and K
+// contributions; the block order is fixed, so results are reproducible.
+void DfPlan::plan_streaming() {
+  const std::size_t budget = opts_.device_budget;
+  const std::size_t nn = static_cast<std::size_t>(n_) * n_;
+  const std::size_t fixed =
+      G_.bytes() + boys_.bytes() +
+      (kSquareMatrices * nn + std::min(exchange_partial_size(n_), 10 * nn) + ldq_ + npair_ +
+       std::max(coulomb_partial_size(npair_, ldq_), static_cast<std::size_t>(n_) * ldq_)) *
+          sizeof(double);
+  if (fixed >= budget) throw OutOfMemory("DF memory budget cannot hold the metric and one contraction tile");
+  const std::size_t avail = budget - fixed;
+  // Row tiles: consecutive shell ranges whose packed rows fit a quarter of what is left.
+  const std::size_t row_cap = avail / 4 / (static_cast<std::size_t>(ldq_) * sizeof(double));
+  const auto& shells = ob_.host_shells;
+  const int nshell = static_cast<int>(shells.size());
+  auto first_fn = [&](int sidx) {
+    return sidx < nshell ? static_cast<std::size_t>(shells[sidx].fn_off) : static_cast<std::size_t>(n_);
+  };
+  auto rows = [&](int s0, int s1) { return pair_count(first_fn(s1)) - pair_count(first_fn(s0)); };
+  tile_shells_.assign(1, 0);
+  std::size_t max_rows = 0;
+  for (int s0 = 0; s0 < nshell;) {
+    int s1 = s0 + 1;
+    while (s1 < nshell && rows(s0, s1 + 1) <= row_cap) ++s1;
+    if (rows(s0, s1) > row_cap)
+      throw OutOfMemory("DF memory budget cannot hold the three-centre rows of one shell");
+    max_rows = std::max(max_rows, rows(s0, s1));
+    tile_shells_.push_back(s1);
+    s0 = s1;
+  }
+  const std::size_t tile_bytes = max_rows * ldq_ * sizeof(double);
+  // Auxiliary block width for the SCF's occupied rank (a wider factor may
+  // narrow a build's blocks, see build_jk_streamed).
+  const std::size_t per_column = (npair_ + static_cast<std::size_t>(n_) * (std::max(nocc_, 1) + 1)) * sizeof(double);
+  const std::size_t q = (avail - tile_bytes) / per_column / 128 * 128;
+  if (q < 128) throw OutOfMemory("DF memory budget cannot hold one auxiliary block of the fitted tensor");
+  qblk_ = static_cast<int>(std::min<std::size_t>(q, ldq_));
+}
+
+void DfPlan::generate_block(int q0, int q1, int ldqb) {
+  const double one = 1.0, zero = 0.0;
+  VQC_HIP(hipMemsetAsync(Bblk_.data(), 0, npair_ * ldqb * sizeof(double), stream_));
+  const int k = g_lower_ ? q1 : naux_;  // G = L^-1 has no entries right of its diagonal
+  const auto& shells = ob_.host_shells;
+  const int nshell = static_cast<int>(shells.size());
+  for (std::size_t t = 0; t + 1 < tile_shells_.size(); ++t) {
+    const int s0 = tile_shells_[t], s1 = tile_shells_[t + 1];
+    const std::size_t f0 = shells[s0].fn_off, f1 = s1 < nshell ? shells[s1].fn_off : n_;
+    const std::size_t row0 = pair_count(f0), rows = pair_count(f1) - row0;
+    compute_three_center(orb_, ob_, al_, boys_.data(), Ablk_.data(), ldq_, opts_.screening * 1e-10, stream_, s0, s1);
+    VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, q1 - q0, static_cast<int>(rows), k,
+                           &one, G_.data() + q0, naux_, Ablk_.data(), ldq_, &zero, Bblk_.data() + row0 * ldqb, ldqb));
+  }
+}
+
+void DfPlan::build_jk_streamed(const double* D, int r, int r_pos, bool exchange) {
+  const std::size_t nn = static_cast<std::size_t>(n_) * n_;
+  const bool do_k = exchange && r > 0;
+  const bool fused_rho = do_k && opts_.coulomb;
+  // The U block holds at most the planned factor width; a wider factor (an
+  // indefinite or high-rank density) is taken in column chunks of that width,
+  // each copied into a zero-padded factor of its own.
+  const int rc = std::min(r, std::max(nocc_, 1));
+  const int qb = qblk_;
+  std::size_t max_rows = 0;
+  const auto& shells = ob_.host_shells;
+  const int nshell = static_cast<int>(shells.size());
+  for (std::size_t t = 0; t + 1 < tile_shells_.size(); ++t) {
+    const int s1 = tile_shells_[t + 1];
+    const std::size_t f0 = shells[tile_shells_[t]].fn_off, f1 = s1 < nshell ? shells[s1].fn_off : n_;
+    max_rows = std::max(max_rows, pair_count(f1) - pair_count(f0));
+  }
+  if (Ablk_.size() < max_rows * ldq_) Ablk_.resize(max_rows * ldq_);
+  if (Bblk_.size() < npair_ * qb) Bblk_.resize(npair_ * qb);
+  if (Jtmp_.size() != nn) {
+    Jtmp_.resize(nn);
+    Ktmp_.resize(nn);
+  }
+  if (do_k && Ublk_.size() < static_cast<std::size_t>(n_) * rc * qb * sizeof(double)) {
+    Ublk_.release();
+    Ublk_.resize(static_cast<std::size_t>(n_) * rc * qb * sizeof(double));
+  }
+  // Column chunks of the factor: [c0, c0 + width), the first `positive` of them +1.
+  struct Chunk {
+    OccupiedOperands ops;
+    int positive;
+  };
+  std::vector<Chunk> chunks;
+  DeviceArray<double> chunk_factor;
+  if (do_k) {
+    if (r <= rc) {
+      ops_.r = r;
+      prepare_occupied(Precision::FP64, ops_, stream_);
+      chunks.push_back({ops_, r_pos});
+    } else {
+      const int ni = occupied_tile(rc);
+      const int ldcc = static_cast<int>(round_up(rc, 32 * ni));
+      const int nchunk = (r + rc - 1) / rc;
+      chunk_factor.resize(static_cast<std::size_t>(nchunk) * n_ * ldcc);
+      VQC_HIP(hipMemsetAsync(chunk_factor.data(), 0, chunk_factor.bytes(), stream_));
+      for (int c = 0; c < nchunk; ++c) {
+        const int c0 = c * rc, width = std::min(rc, r - c0);
+        double* dst = chunk_factor.data() + static_cast<std::size_t>(c) * n_ * ldcc;
+        VQC_HIP(hipMemcpy2DAsync(dst, ldcc * sizeof(double), c64_.data() + c0, ldc_ * sizeof(double),
+                                 width * sizeof(double), n_, hipMemcpyDeviceToDevice, stream_));
+        OccupiedOperands o = ops_;
+        o.r = width;
+        o.ni = ni;
+        o.ldc = ldcc;
+        o.c64 = dst;
+        chunks.push_back({o, std::clamp(r_pos - c0, 0, width)});
+      }
+    }
+  }
+  VQC_HIP(hipMemsetAsync(J_.data(), 0, nn * sizeof(double), stream_));
+  VQC_HIP(hipMemsetAsync(K_.data(), 0, nn * sizeof(double), stream_));
+  if (opts_.coulomb && !fused_rho) pack_density(D, n_, dpacked_.data(), stream_);
+  const double one = 1.0;
+  for (int q0 = 0; q0 < naux_; q0 += qb) {
+    const int q1 = std::min(q0 + qb, naux_);
+    const int ldqb = static_cast<int>(round_up(q1 - q0, 128));
+    generate_block(q0, q1, ldqb);
+    for (std::size_t c = 0; c < chunks.size(); ++c) {
+      Chunk& ch = chunks[c];
+      ch.ops.ustride = u_row_stride(Precision::FP64, ch.ops.r, ldqb);
+      occupied_u(Precision::FP64, Bblk_.data(), ch.ops, ldqb, Ublk_.data(), D,
+                 fused_rho && c == 0 ? rho_partial_.data() : nullptr, stream_);
+      exchange_syrk(Precision::FP64, Ublk_.data(), ch.ops.ustride, n_, ch.ops.r, ch.positive, ldqb, Ktmp_.data(),
+                    syrk_partial_.data(), stream_, syrk_partial_.size());
+      VQC_BLAS(rocblas_daxpy(blas_, static_cast<int>(nn), &one, Ktmp_.data(), 1, K_.data(), 1));
+    }
+    if (opts_.coulomb) {
+      if (fused_rho)
+        rho_from_partials(rho_partial_.data(), n_, ldqb, rho_.data(), stream_);
+      else
+        coulomb_rho(Precision::FP64, Bblk_.data(), dpacked_.data(), npair_, ldqb, rho_.data(), rho_partial_.data(),
+                    stream_);
+      coulomb_matrix(Precision::FP64, Bblk_.data(), rho_.data(), npair_, ldqb, n_, Jtmp_.data(), stream_);
+      VQC_BLAS(rocblas_daxpy(blas_, static_cast<int>(nn), &one, Jtmp_.data(), 1, J_.data(), 1));
+    }
   }
 }
 
@@ -448,6 +663,7 @@ JkResult DfPlan::evaluate(const std::vector<double>& density, std::vector<double
   if (!opts_.coulomb) std::fill(out.J.begin(), out.J.end(), 0.0);
   if (!opts_.exchange) std::fill(out.K.begin(), out.K.end(), 0.0);
   if (gradient) {
+    require_resident("the fixed-density derivative");
     const std::size_t ng = orb_.atoms.size() * 3;
     DeviceArray<double> g(ng);
     VQC_HIP(hipMemsetAsync(g.data(), 0, g.bytes(), stream_));
@@ -485,6 +701,7 @@ void DfPlan::evaluate_occupied_into(const double* C, int nocc, double* J, double
   converged_state_ = false;
   if (nocc < 1 || nocc > n_) throw std::invalid_argument("occupied column count out of range");
   if (!C || !J || !K || !scratch) throw std::invalid_argument("null device buffer");
+  require_resident("the allocation-free occupied build");
   const Precision p = prec_;
   if (p != Precision::FP64 && !B32_.size())
     throw std::logic_error("the reduced-precision copy of B is made by set_precision");
@@ -541,6 +758,7 @@ std::array<double, 5> DfPlan::benchmark_occupied(const std::vector<double>& occu
   converged_state_ = false;
   VQC_HIP(hipSetDevice(device_));
   if (repeat == 0) throw std::invalid_argument("repeat must be positive");
+  require_resident("the kernel benchmark");
   if (nocc <= 0 || occupied.size() != static_cast<std::size_t>(n_) * nocc)
     throw std::invalid_argument("occupied factor shape mismatch");
   ensure_factor(nocc);
@@ -602,6 +820,7 @@ std::vector<double> DfPlan::overlap() const {
 
 std::vector<double> DfPlan::converged_forces() {
   VQC_HIP(hipSetDevice(device_));
+  require_resident("nuclear forces");
   if (!converged_state_) throw std::logic_error("forces need a converged solve on this plan");
   converged_state_ = false;  // the gradient overwrites U_
   if (converged_unrestricted_) {
@@ -629,6 +848,7 @@ std::vector<double> DfPlan::converged_forces() {
 
 ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial_density, bool forces) {
   VQC_HIP(hipSetDevice(device_));
+  if (forces) require_resident("nuclear forces");
   converged_state_ = false;
   converged_unrestricted_ = false;
   if (orb_.multiplicity != 1 || orb_.electrons % 2 != 0)

@@ -19,6 +19,10 @@ struct PlanOptions {
   double metric_threshold = 1e-10;
   double cj = 1.0, ck = -0.5;
   bool coulomb = true, exchange = true;
+  // Device bytes the plan may hold (0: no limit).  When the resident fitted
+  // tensor does not fit, the plan keeps only the metric factor and
+  // regenerates B in auxiliary blocks for every Fock build (streamed plan).
+  std::size_t device_budget = 0;
 };
 
 struct ScfOptions {
@@ -69,6 +73,10 @@ class DfPlan {
   int naux() const { return naux_; }
   std::size_t npair() const { return npair_; }
   int rank() const { return rank_; }
+  // True when B is regenerated per Fock build instead of held on the device.
+  bool streamed() const { return streamed_; }
+  int auxiliary_block() const { return streamed_ ? qblk_ : ldq_; }
+  int row_tiles() const { return streamed_ ? static_cast<int>(tile_shells_.size()) - 1 : 1; }
   bool cholesky() const { return cholesky_; }
   double condition() const { return condition_; }
   double nuclear_repulsion() const { return enuc_; }
@@ -115,7 +123,16 @@ class DfPlan {
 
  private:
   void setup(const System& aux);
-  void factor_metric(DeviceArray<double>& M, DeviceArray<double>& A);
+  // Factor the metric and apply it to A (npair x ldq, in place), or, with A
+  // null, keep the factor as G_ for streamed builds.
+  void factor_metric(DeviceArray<double>& M, double* A);
+  // Streamed plans (device_budget below the resident footprint).
+  std::size_t resident_bytes() const;
+  void plan_streaming();
+  // B[:, q0:q1] (npair rows, stride ldqb, padding columns zero) into Bblk_.
+  void generate_block(int q0, int q1, int ldqb);
+  void build_jk_streamed(const double* D, int r, int r_pos, bool exchange);
+  void require_resident(const char* what) const;
   void ensure_reduced();
   void ensure_factor(int r);
   // J/K on the device: D (n x n) and the factor already in c64_ with r
@@ -183,6 +200,16 @@ class DfPlan {
   int factor_cap_ = 0, ldc_ = 0, ni_ = 1;
   OccupiedOperands ops_;
   DeviceArray<int> flag_;
+  // Streamed plans: G_ (naux x naux, column-major) maps A to B (B^T = G A^T;
+  // the inverse Cholesky factor, lower triangular, or the symmetric X), the
+  // row tiles of the three-centre regeneration (shell ranges) and the
+  // per-build block buffers.
+  bool streamed_ = false;
+  bool g_lower_ = false;
+  DeviceArray<double> G_, Ablk_, Bblk_, Jtmp_, Ktmp_;
+  DeviceArray<char> Ublk_;
+  std::vector<int> tile_shells_;  // shell boundaries of the row tiles
+  int qblk_ = 0;                  // auxiliary columns per block (multiple of 128)
 };
 
 }  // namespace vqc

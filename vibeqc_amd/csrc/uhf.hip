@@ -65,6 +65,10 @@ std::pair<int, int> spin_occupations(const System& sys) {
 }  // namespace
 
 void DfPlan::coulomb_only(Precision p, const double* D) {
+  if (streamed_) {
+    build_jk_streamed(D, 0, 0, false);
+    return;
+  }
   if (p != Precision::FP64) ensure_reduced();
   const void* B = p == Precision::FP64 ? static_cast<const void*>(B64_.data()) : static_cast<const void*>(B32_.data());
   pack_density(D, n_, dpacked_.data(), stream_);
@@ -137,6 +141,7 @@ UhfJk DfPlan::evaluate_unrestricted(const std::vector<double>& alpha, const std:
   }
   sync();
   if (gradient) {
+    require_resident("the fixed-density derivative");
     const std::size_t ng = orb_.atoms.size() * 3;
     DeviceArray<double> g(ng);
     VQC_HIP(hipMemsetAsync(g.data(), 0, g.bytes(), stream_));
@@ -150,6 +155,7 @@ UhfJk DfPlan::evaluate_unrestricted(const std::vector<double>& alpha, const std:
 
 ScfOutcome DfPlan::solve_unrestricted(const ScfOptions& o, const std::vector<double>* initial_density, bool forces) {
   VQC_HIP(hipSetDevice(device_));
+  if (forces) require_resident("nuclear forces");
   converged_state_ = false;
   converged_unrestricted_ = false;
   if (!opts_.coulomb || !opts_.exchange || opts_.cj != 1.0 || opts_.ck != -1.0)

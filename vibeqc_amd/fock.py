@@ -245,6 +245,14 @@ class FockPlan:
             "device_bytes": out[5],
         }
 
+    def streaming(self) -> dict:
+        """Whether the fitted tensor is regenerated per build to honour
+        ``device_budget_bytes`` (``streamed``), with the auxiliary block width
+        and the three-centre row tiles of each block."""
+        out = (ct.c_uint64 * 3)()
+        self._check(self._lib.vibeqc_fock_plan_streaming(self._handle, out))
+        return {"streamed": bool(out[0]), "auxiliary_block": out[1], "row_tiles": out[2]}
+
     def last_solve_stats(self) -> dict:
         """Statistics of the last solve(): iterations, iterations with
         reduced-precision Fock builds, new densities from purification."""
