@@ -102,19 +102,24 @@ def test_torch_batched_energy_backward():
 
 
 def test_batch_is_reproducible(monkeypatch):
-    """Repeated executions are bitwise identical, with one worker and with
-    concurrent workers (their SCFs overlap; setups and forces run alone)."""
+    """Energies and forces of repeated executions are bitwise identical, with
+    one worker and with concurrent workers (their SCFs overlap; setups and
+    forces run one system at a time)."""
     from vibeqc_amd import Calculator
     from vibeqc_amd.geometry import water_cluster
 
     rng = np.random.default_rng(9)
-    sizes = rng.integers(1, 6, size=16)
+    sizes = rng.integers(1, 6, size=48)
     systems = [water_cluster(int(n), 3.0 * n ** (1.0 / 3.0), seed=300 + k) for k, n in enumerate(sizes)]
+
+    def run():
+        res = batch.execute()
+        return np.asarray(res.energies), np.concatenate([np.asarray(it.forces).ravel() for it in res.items])
+
     with Calculator().prepare_batch(systems, warm_start=False) as batch:
         monkeypatch.setenv("VIBEQC_BATCH_WORKERS", "1")
-        serial = [batch.execute(properties=("energy",)).energies for _ in range(2)]
+        serial = [run() for _ in range(2)]
         monkeypatch.delenv("VIBEQC_BATCH_WORKERS")
-        concurrent = [batch.execute(properties=("energy",)).energies for _ in range(2)]
-    assert np.array_equal(serial[0], serial[1])
-    for e in concurrent:
-        assert np.array_equal(e, serial[0])
+        concurrent = [run() for _ in range(3)]
+    for e, f in serial[1:] + concurrent:
+        assert np.array_equal(e, serial[0][0]) and np.array_equal(f, serial[0][1])

@@ -163,11 +163,13 @@ std::vector<BatchItemOutcome> BatchPlan::execute(const std::vector<const std::ve
   // B), then their SCFs run concurrently, one worker per stream, and forces
   // are evaluated one system at a time.  Setup and derivative-integral
   // kernels therefore never share the device with another system's work.
-  // When setups ran on several hardware queues at once, identical device
-  // inputs (checksummed) produced different metric, three-centre and
-  // Cholesky results (energies off by up to 0.5 Eh); the same parallel setups
-  // funnelled through one hardware queue were bitwise reproducible.  With the
-  // phases separated, repeated executions are bitwise reproducible.
+  // Concurrent SCFs are bitwise reproducible.  Setups that overlap other
+  // members' work are not: with setups on several hardware queues at once,
+  // identical device inputs (checksummed) gave different metric, three-centre
+  // and Cholesky results, and with every setup kernel free of per-lane
+  // scratch memory, setups overlapping other members' SCFs still turned one
+  // energy of 48 into NaN and moved others by up to 1e-6 Eh.  The cause is
+  // not established, so setups stay serial.
   const std::size_t wave = static_cast<std::size_t>(workers) * 4;
   std::vector<std::unique_ptr<DfPlan>> plans(count);
   auto setup_item = [&](std::size_t k) {
