@@ -1,9 +1,17 @@
-"""The oracle's metric factorisation against the reference's own code.
+"""The oracle's basis handling and metric factorisation against the
+reference's own code.
 
-oracle/_ref/libref_metric.so is the reference's factor_density_fitting_metric
+oracle/_ref/libref.so holds the reference's Cartesian order, AO expansions and
+shell normalisation (src/molecule/basis.cpp) and its factor_density_fitting_metric
 (src/integrals/density_fitting_metric.cpp:101-152 with its in-house linear
-algebra, src/tensor/cpu_linalg.cpp) compiled from the reference sources in
-place by oracle/Makefile (`make -C oracle ref`).  oracle.metric_inverse_sqrt
+algebra, src/tensor/cpu_linalg.cpp), compiled from the reference sources in
+place by oracle/Makefile (`make -C oracle ref`).
+
+oracle.cart_components and oracle.spherical_rows must reproduce the
+reference's component order and real-solid-harmonic rows (s through g,
+1e-14), and oracle.normalise_shell its normalised coefficients for every shell
+of cc-pVDZ and cc-pVDZ-JKFIT (H, N, O) and def2-SVP (H to Ar) (1e-13
+relative).  oracle.metric_inverse_sqrt
 must give the same inverse square root (to 1e-15 times the condition number
 of its largest element: two FP64 eigensolvers differ by that much), the same
 effective rank and the same condition number (1e-8 relative) on the
@@ -24,7 +32,7 @@ import pytest
 from _cases import POSTHF, ccpvdz_case
 from conftest import ROOT, golden_npz
 
-LIB = ROOT / "oracle" / "_ref" / "libref_metric.so"
+LIB = ROOT / "oracle" / "_ref" / "libref.so"
 pytestmark = pytest.mark.skipif(not LIB.exists(), reason="oracle/_ref not built (no reference tree)")
 
 
@@ -40,6 +48,43 @@ def _reference_factor(M, threshold=1e-10):
     rank, cond = ct.c_int(), ct.c_double()
     assert f(M.ctypes.data_as(dp), n, threshold, X.ctypes.data_as(dp), ct.byref(rank), ct.byref(cond)) == 0
     return X, rank.value, cond.value
+
+
+def test_cartesian_order_and_ao_expansions(oracle):
+    lib = ct.CDLL(str(LIB))
+    for l in range(5):
+        comps = (ct.c_int * (3 * 15))()
+        count = lib.ref_cartesian_components(l, comps, 15)
+        assert [tuple(comps[3 * c : 3 * c + 3]) for c in range(count)] == list(oracle.cart_components(l))
+        ncart = count
+        for spherical in (0, 1):
+            out = np.zeros((ncart, ncart))
+            rows = lib.ref_ao_expansion(l, spherical, out.ctypes.data_as(ct.POINTER(ct.c_double)), ncart, ncart)
+            ref = out[:rows]
+            mine = oracle.spherical_rows(l) if spherical else np.eye(ncart)
+            assert ref.shape == mine.shape, (l, spherical)
+            assert np.abs(ref - mine).max() < 1e-14, (l, spherical)
+
+
+def test_shell_normalisation(oracle):
+    from vibeqc_amd.basis import Atom, shells_for
+
+    lib = ct.CDLL(str(LIB))
+    dp = ct.POINTER(ct.c_double)
+    elements = {"cc-pvdz": (1, 7, 8), "cc-pvdz-jkfit": (1, 7, 8), "def2-svp": tuple(range(1, 19))}
+    checked = 0
+    for basis, zs in elements.items():
+        atoms = tuple(Atom(z, (0.0, 0.0, 2.0 * k)) for k, z in enumerate(zs))
+        for sh in shells_for(basis, atoms):
+            e = np.array([p.exponent for p in sh.primitives])
+            c = np.array([p.coefficient for p in sh.primitives])
+            out = np.empty_like(c)
+            assert lib.ref_normalize_shell(sh.l, len(e), e.ctypes.data_as(dp), c.ctypes.data_as(dp),
+                                           out.ctypes.data_as(dp)) == 0
+            _, mine = oracle.normalise_shell(oracle.Shell(0, sh.l, tuple(e), tuple(c)))
+            assert np.abs(mine - out).max() <= 1e-13 * np.abs(out).max(), (basis, sh.l)
+            checked += 1
+    assert checked > 100
 
 
 def _metrics(oracle):
