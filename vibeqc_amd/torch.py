@@ -30,7 +30,7 @@ class _EnergyFunction(torch.autograd.Function):
         want = ("energy", "forces") if coordinates.requires_grad else ("energy",)
         result = calculator.singlepoint(atoms, charge=charge, multiplicity=multiplicity, properties=want)
         if result.forces is not None:
-            ctx.save_for_backward(torch.as_tensor(result.forces, dtype=coordinates.dtype, device=coordinates.device))
+            ctx.save_for_backward(torch.tensor(np.array(result.forces), dtype=coordinates.dtype, device=coordinates.device))
         return coordinates.new_tensor(result.energy)
 
     @staticmethod
@@ -112,7 +112,7 @@ class _BatchedEnergyFunction(torch.autograd.Function):
             xyz = [np.asarray([p for _, p in s]) for s in systems]
             result = prepared_batch.execute(xyz, strict=True, properties=want)
         forces = [
-            torch.as_tensor(it.forces, dtype=c.dtype, device=c.device) if it.forces is not None else None
+            torch.tensor(np.array(it.forces), dtype=c.dtype, device=c.device) if it.forces is not None else None
             for it, c in zip(result.items, coords)
         ]
         ctx.forces = forces
