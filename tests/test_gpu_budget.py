@@ -111,3 +111,31 @@ def test_calculator_energy_under_budget(resident):
     r = Calculator(density_fitting_memory_budget_bytes=24 * MB, energy_tolerance=1e-11,
                    density_tolerance=1e-9).singlepoint(case.atoms, properties=("energy",))
     assert abs(r.energy - ref["scf"].energy) < 1e-10
+
+
+def test_streamed_unrestricted_plan():
+    """(H2O)4+ doublet: the UHF SCF of a plan streamed under 24 MB equals the
+    resident plan's (1e-10 Eh), and so do its unrestricted J/K (1e-10
+    relative)."""
+    from dataclasses import replace
+
+    from vibeqc_amd import FockBuildSpec, FockPlan
+
+    case, _ = ccpvdz_case("ccpvdz_water4")
+    cation = replace(case, charge=1, multiplicity=2)
+    spec = FockBuildSpec.hf("unrestricted")
+    rng = np.random.default_rng(2)
+    Ca, Cb = rng.normal(size=(96, 20)) * 0.3, rng.normal(size=(96, 19)) * 0.3
+    D = np.stack([Ca @ Ca.T, Cb @ Cb.T])
+    out = {}
+    for budget in (0, 24 * MB):
+        with FockPlan(cation.orbital(), spec, auxiliary=cation.auxiliary(), device_budget_bytes=budget) as plan:
+            assert plan.streaming()["streamed"] == (budget > 0)
+            ev = plan.evaluate(D)
+            r = plan.solve(**TIGHT)
+        out[budget] = (ev, r.energy)
+    (e0, E0), (e1, E1) = out[0], out[24 * MB]
+    print(f"(H2O)4+ UHF streamed - resident: {E1 - E0:.1e} Eh")
+    assert abs(E1 - E0) < 1e-10
+    for a, b in ((e1.coulomb, e0.coulomb), (e1.exchange[0], e0.exchange[0]), (e1.exchange[1], e0.exchange[1])):
+        assert _rel(a, b) < 1e-10

@@ -171,3 +171,49 @@ def test_not_converged_status(water4):
         # The plan stays usable after a failure.
         r = plan.solve()
         assert np.isfinite(r.energy)
+
+
+def test_cartesian_f_shells_and_open_shell_f(oracle):
+    """The f_heh orbital and fitting sets in the Cartesian representation
+    (10-component f): J/K of a random density against the oracle (1e-10), the
+    closed-shell SCF energy (1e-9 Eh) and forces (1e-8 Eh/bohr) against the
+    oracle's; and neutral HeH (doublet, UHF) with the same basis against the
+    UHF oracle's energy (1e-9 Eh) and gradient (1e-8 Eh/bohr)."""
+    from dataclasses import replace
+
+    from vibeqc_amd import Calculator, FockPlan
+
+    case, _, _ = posthf_case("f_heh")
+    cart = replace(case, representation="cartesian")
+    P = cart.oracle_problem(oracle)
+    n = P.S.shape[0]
+    assert n == 1 + 10 + 1
+    rng = np.random.default_rng(12)
+    C = rng.normal(size=(n, 2)) * 0.4
+    D = 2 * C @ C.T
+    J, K = oracle.jk(P.B, D)
+    tight = dict(energy_tolerance=1e-12, density_tolerance=1e-10)
+    with FockPlan(cart.orbital(), auxiliary=cart.auxiliary()) as plan:
+        ev = plan.evaluate(D)
+        r = plan.solve(compute_forces=True, **tight)
+    assert np.abs(ev.coulomb - J).max() < 1e-10 and np.abs(ev.exchange - K).max() < 1e-10
+    sh, ax = cart.oracle_shells(oracle), cart.oracle_shells(oracle, True)
+    ref = oracle.df_rhf(P, **tight)
+    g = oracle.df_rhf_gradient(cart.z, cart.xyz, sh, False, ax, False, P, ref)
+    print(f"Cartesian f_heh: dE {r.energy - ref.energy:.2e}, max dF {np.abs(np.asarray(r.forces) + g).max():.2e}")
+    assert abs(r.energy - ref.energy) < 1e-9
+    assert np.abs(np.asarray(r.forces) + g).max() < 1e-8
+
+    # Neutral HeH doublet (three electrons), UHF with the spherical f basis.
+    open_shell = replace(case, charge=0, multiplicity=2)
+    Po = open_shell.oracle_problem(oracle)
+    na, nb = oracle.spin_occupations(3, 2)
+    uref = oracle.df_uhf(Po, na, nb, **tight)
+    sho, axo = open_shell.oracle_shells(oracle), open_shell.oracle_shells(oracle, True)
+    gu = oracle.df_uhf_gradient(open_shell.z, open_shell.xyz, sho, True, axo, True, Po, uref, (na, nb))
+    res = Calculator(method="uhf", basis=case.shells, auxiliary_basis=case.aux_shells,
+                     basis_representation="real_spherical", energy_tolerance=1e-12,
+                     density_tolerance=1e-10).singlepoint(case.atoms, charge=0, multiplicity=2)
+    print(f"HeH doublet UHF: dE {res.energy - uref.energy:.2e}, max dF {np.abs(np.asarray(res.forces) + gu).max():.2e}")
+    assert abs(res.energy - uref.energy) < 1e-9
+    assert np.abs(np.asarray(res.forces) + gu).max() < 1e-8
