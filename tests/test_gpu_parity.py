@@ -152,14 +152,35 @@ def test_mixed_precision_scf_within_one_microhartree(water4):
         assert abs(r.energy - ref.energy) < 1e-6
 
 
-def test_asymmetric_density_rejected_and_outputs_untouched(water4):
+def test_nonsymmetric_density(oracle, water4):
+    """A nonsymmetric density (as the reference's raw contractions accept,
+    fock_build.hpp:173): J depends on its symmetric part, K = sum_Q B_Q D B_Q
+    keeps its orientation — both against the oracle (1e-10 relative), also for
+    an unrestricted plan's spin densities.  The derivative still needs a
+    symmetric density (INVALID_ARGUMENT, outputs untouched)."""
+    from vibeqc_amd import FockBuildSpec
+
     case, P, C = water4
     n = P.S.shape[0]
-    D = 2 * C @ C.T
-    D[0, 1] += 1e-3
+    rng = np.random.default_rng(21)
+    D = 2 * C @ C.T + 0.05 * rng.normal(size=(n, n))
+    J, K = oracle.jk(P.B, D)
+    assert np.abs(K - K.T).max() > 1e-3  # the antisymmetric part reaches K
     with _plan(case) as plan:
+        ev = plan.evaluate(D)
+    print(f"nonsymmetric D: rel dJ {np.abs(ev.coulomb - J).max() / np.abs(J).max():.1e}, "
+          f"rel dK {np.abs(ev.exchange - K).max() / np.abs(K).max():.1e}")
+    assert np.abs(ev.coulomb - J).max() < 1e-10 * np.abs(J).max()
+    assert np.abs(ev.exchange - K).max() < 1e-10 * np.abs(K).max()
+    Db = 0.5 * D.T + 0.02 * rng.normal(size=(n, n))
+    with FockPlan(case.orbital(), FockBuildSpec.hf("unrestricted"), auxiliary=case.auxiliary()) as plan:
+        ev = plan.evaluate(np.stack([0.5 * D, Db]))
+    for dev, dens in ((ev.exchange[0], 0.5 * D), (ev.exchange[1], Db)):
+        Ks = oracle.jk(P.B, dens)[1]
+        assert np.abs(dev - Ks).max() < 1e-10 * np.abs(Ks).max()
+    with _plan(case) as plan:  # FockBuildSpec.hf(): derivative order 1
         with pytest.raises(_native.NativeError) as err:
-            plan.evaluate(D)
+            plan.evaluate(D, derivative=True)
         assert err.value.status == 1
 
 

@@ -643,17 +643,114 @@ std::vector<double> symmetrised_density(const std::vector<double>& d, int n) {
   return out;
 }
 
+bool split_density(const std::vector<double>& d, int n, std::vector<double>& sym, std::vector<double>& anti) {
+  double scale = 0.0, asym = 0.0;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      const double a = d[static_cast<std::size_t>(i) * n + j], b = d[static_cast<std::size_t>(j) * n + i];
+      if (!std::isfinite(a)) throw std::invalid_argument("density must be finite");
+      scale = std::max(scale, std::abs(a));
+      asym = std::max(asym, std::abs(a - b));
+    }
+  const bool nonsymmetric = asym > 1e-14 * scale;
+  sym.resize(d.size());
+  if (nonsymmetric) anti.resize(d.size());
+  else anti.clear();
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      const double a = d[static_cast<std::size_t>(i) * n + j], b = d[static_cast<std::size_t>(j) * n + i];
+      sym[static_cast<std::size_t>(i) * n + j] = 0.5 * (a + b);
+      if (nonsymmetric) anti[static_cast<std::size_t>(i) * n + j] = 0.5 * (a - b);
+    }
+  return nonsymmetric;
+}
+
+void DfPlan::add_general_exchange(const double* A, double* K) {
+  require_resident("a nonsymmetric density");
+  const int n = n_;
+  const std::size_t nn = static_cast<std::size_t>(n) * n;
+  const double one = 1.0, zero = 0.0;
+  // The row-major array of A read column-major is A^T: A^T A = (A_cm)(A_cm)^T.
+  DeviceArray<double> S(nn), V(nn), w(n), e(n), AV(nn);
+  DeviceArray<rocblas_int> info(1);
+  VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_transpose, n, n, n, &one, A, n, A, n, &zero,
+                         S.data(), n));
+  copy_device(V.data(), S.data(), nn, stream_);
+  VQC_BLAS(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_lower, n, V.data(), n, w.data(), e.data(),
+                            info.data()));
+  std::vector<double> hw(n);
+  rocblas_int hinfo = 0;
+  w.download(hw.data(), n, stream_);
+  info.download(&hinfo, 1, stream_);
+  sync();
+  if (hinfo != 0) throw NumericalFailure("density eigensolver did not converge");
+  const double largest = std::max(hw.back(), 0.0);
+  std::vector<int> idx;
+  std::vector<double> sigma(n, 0.0), inv_sigma(n, 0.0);
+  for (int k = n - 1; k >= 0; --k)
+    if (hw[k] > 1e-28 * largest && hw[k] > 0.0) {
+      idx.push_back(k);
+      sigma[k] = std::sqrt(hw[k]);
+      inv_sigma[k] = 1.0 / sigma[k];
+    }
+  const int r = static_cast<int>(idx.size());
+  if (r == 0) return;
+  // A V = (A_cm)^T V: columns A v_k.
+  VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, n, n, n, &one, A, n, V.data(), n,
+                         &zero, AV.data(), n));
+  ensure_factor(r);
+  const int ni = occupied_tile(r);
+  const int ldc = static_cast<int>(round_up(r, 32 * ni));
+  DeviceArray<double> cp(static_cast<std::size_t>(n) * ldc), cq(static_cast<std::size_t>(n) * ldc), ws(n), wi(n);
+  DeviceArray<int> didx;
+  didx.upload(idx, stream_);
+  ws.upload(sigma, stream_);
+  wi.upload(inv_sigma, stream_);
+  // Columns: P_k = A v_k / sqrt(s_k), Q_k = v_k sqrt(s_k).
+  factor_from_eigen(AV.data(), wi.data(), didx.data(), n, r, ldc, cp.data(), stream_);
+  factor_from_eigen(V.data(), ws.data(), didx.data(), n, r, ldc, cq.data(), stream_);
+  OccupiedOperands op;
+  op.n = n;
+  op.r = r;
+  op.ldc = ldc;
+  op.ni = ni;
+  op.ldct = static_cast<int>(round_up(n, 32));
+  op.ustride = u_row_stride(Precision::FP64, r, ldq_);
+  op.c64 = cp.data();
+  OccupiedOperands oq = op;
+  oq.c64 = cq.data();
+  DeviceArray<double> Uq(static_cast<std::size_t>(n) * op.ustride), M(nn);
+  occupied_u(Precision::FP64, B64_.data(), op, ldq_, U_.data(), nullptr, nullptr, stream_);
+  occupied_u(Precision::FP64, B64_.data(), oq, ldq_, Uq.data(), nullptr, nullptr, stream_);
+  // Row-major M[mu][lam] = sum_k U_P[mu][k] U_Q[lam][k]: column-major M^T = U_Q^T U_P.
+  const int kd = static_cast<int>(static_cast<std::size_t>(r) * ldq_);
+  VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, n, n, kd, &one, Uq.data(),
+                         static_cast<int>(op.ustride), static_cast<const double*>(static_cast<const void*>(U_.data())),
+                         static_cast<int>(op.ustride), &zero, M.data(), n));
+  VQC_BLAS(rocblas_daxpy(blas_, static_cast<int>(nn), &one, M.data(), 1, K, 1));
+  sync();
+}
+
 JkResult DfPlan::evaluate(const std::vector<double>& density, std::vector<double>* gradient) {
   converged_state_ = false;
   VQC_HIP(hipSetDevice(device_));
   const std::size_t nn = static_cast<std::size_t>(n_) * n_;
   if (density.size() != nn) throw std::invalid_argument("density shape mismatch");
-  const std::vector<double> sym = symmetrised_density(density, n_);
+  std::vector<double> sym, anti;
+  const bool nonsymmetric = split_density(density, n_, sym, anti);
+  if (nonsymmetric && gradient)
+    throw std::invalid_argument("the fixed-density derivative requires a symmetric density");
   ensure_factor(1);
   D_.upload(sym.data(), nn, stream_);
   int packed = opts_.exchange ? factorise_density(D_.data()) : 0;
   const int r_pos = packed & 0xffff, r = packed >> 16;
   build_jk(gradient ? Precision::FP64 : prec_, D_.data(), r, r_pos);
+  if (nonsymmetric && opts_.exchange) {
+    // J depends on the symmetric part only; K gains sum_Q B_Q D_a B_Q.
+    DeviceArray<double> Da(nn);
+    Da.upload(anti.data(), nn, stream_);
+    add_general_exchange(Da.data(), K_.data());
+  }
   JkResult out;
   out.J.resize(nn);
   out.K.resize(nn);

@@ -51,6 +51,11 @@ struct JkResult {
 // largest element (density-fitted K is built from a factor of D); throws
 // std::invalid_argument otherwise.
 std::vector<double> symmetrised_density(const std::vector<double>& d, int n);
+// Symmetric and antisymmetric parts of a finite n x n density (the reference
+// contracts raw, possibly nonsymmetric densities, fock_build.hpp:173); returns
+// false, with `anti` left empty, when D is symmetric to 1e-14 of its largest
+// element.  Throws std::invalid_argument for non-finite entries.
+bool split_density(const std::vector<double>& d, int n, std::vector<double>& sym, std::vector<double>& anti);
 
 // Unrestricted raw J (of the total density) and K of each spin density.
 struct UhfJk {
@@ -153,6 +158,12 @@ class DfPlan {
   void contract_df_gradient(DeviceArray<double>& WA, DeviceArray<double>& WM, double* grad);
   // J of D alone (rho_ and J_).
   void coulomb_only(Precision p, const double* D);
+  // K += sum_Q B_Q A B_Q for a device n x n matrix A (row-major, in FP64):
+  // A = P Q^T from the eigenpairs of A^T A (P = A V diag(s^-1/2),
+  // Q = V diag(s^1/2), s the singular values), U_P and U_Q by the occupied
+  // U kernel and K += U_P U_Q^T as one GEMM.  Used for the antisymmetric part
+  // of a nonsymmetric density; overwrites the occupied-factor workspaces.
+  void add_general_exchange(const double* A, double* K);
   // UHF two-electron gradient of device spin densities; with orbitals
   // non-null, orbitals[s] (column-major n x n) holds D_s's factor in its
   // first occupied[s] columns, else D_s is eigen-decomposed.

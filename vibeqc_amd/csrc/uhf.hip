@@ -114,7 +114,10 @@ UhfJk DfPlan::evaluate_unrestricted(const std::vector<double>& alpha, const std:
   VQC_HIP(hipSetDevice(device_));
   const std::size_t nn = static_cast<std::size_t>(n_) * n_;
   if (alpha.size() != nn || beta.size() != nn) throw std::invalid_argument("density shape mismatch");
-  const std::vector<double> a = symmetrised_density(alpha, n_), b = symmetrised_density(beta, n_);
+  std::vector<double> a, b, anti[2];
+  const bool nonsymmetric[2] = {split_density(alpha, n_, a, anti[0]), split_density(beta, n_, b, anti[1])};
+  if ((nonsymmetric[0] || nonsymmetric[1]) && gradient)
+    throw std::invalid_argument("the fixed-density derivative requires symmetric densities");
   ensure_factor(1);
   DeviceArray<double> Da(nn), Db(nn);
   Da.upload(a.data(), nn, stream_);
@@ -136,6 +139,12 @@ UhfJk DfPlan::evaluate_unrestricted(const std::vector<double>& alpha, const std:
       copy_device(D_.data(), spin[s], nn, stream_);
       const int packed = factorise_density(D_.data());
       build_jk(p, D_.data(), packed >> 16, packed & 0xffff);
+      if (nonsymmetric[s]) {
+        // K of the spin density's antisymmetric part (J sees the symmetric parts only).
+        DeviceArray<double> A(nn);
+        A.upload(anti[s].data(), nn, stream_);
+        add_general_exchange(A.data(), K_.data());
+      }
       K_.download(dest[s]->data(), nn, stream_);
     }
   }

@@ -300,7 +300,7 @@ class FockPlan:
 
     # -- operations ----------------------------------------------------------
     def evaluate(self, density, *, derivative: bool = False) -> FockEvaluation:
-        """Raw J/K, Fock (with Hcore) and energies at a fixed symmetric density:
+        """Raw J/K, Fock (with Hcore) and energies at a fixed density:
         [AO, AO] for restricted spin, [2, AO, AO] (alpha, beta) for unrestricted.
 
         With ``derivative=True`` (plan created with derivative_order 1) the
