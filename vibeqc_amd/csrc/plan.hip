@@ -1000,7 +1000,14 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   const bool sp2_usable = nocc > 0 && nocc < n && pivoted_cholesky_fits(n, nocc);
   DeviceArray<double> G(sp2_usable ? static_cast<std::size_t>(n) * nocc : 1), sp2_state(kSp2StateSize),
       sp2_partial(sp2_partial_size(n));
-  DeviceArray<int> sp2_status(1);
+  DeviceArray<int> sp2_status(1), proj_status(1);
+  // The previous iteration's projector factor G: once one was accepted, the
+  // next is tried as a projection of it (a few GEMMs) before the
+  // single-workgroup pivoted Cholesky.
+  bool have_factor = false;
+  DeviceArray<double> projA(sp2_usable ? static_cast<std::size_t>(n) * nocc : 1),
+      projM(sp2_usable ? static_cast<std::size_t>(nocc) * nocc : 1);
+  DeviceArray<rocblas_int> proj_info(1);
   int sp2_steps = 0;
   double sp2_lo = 0.0, sp2_hi = 0.0;
   const double dnocc = static_cast<double>(nocc);
@@ -1034,7 +1041,10 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
     gemm(rocblas_operation_none, rocblas_operation_none, n, X.data(), n, T1.data(), n, T2.data());
     bool purified = false;
     if (!canonical && sp2_usable && purify()) {
-      pivoted_cholesky(V.data(), n, nocc, G.data(), sp2_status.data(), stream_);
+      if (have_factor)
+        projected_factor(blas_, V.data(), n, nocc, G.data(), projA.data(), projM.data(), proj_info.data(),
+                         proj_status.data(), stream_);
+      pivoted_cholesky(V.data(), n, nocc, G.data(), sp2_status.data(), stream_, have_factor ? proj_status.data() : nullptr);
       VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, n, nocc, n, &one, X.data(), n,
                              G.data(), n, &zero, W.data(), n));
       purified = true;
@@ -1047,9 +1057,11 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
                            c64_.data(), ldc_, c64_.data(), ldc_, &zero, Dout, n));
     return purified;
   };
+  // Only an accepted factor seeds the next iteration's projection.
   auto purified_ok = [&]() {
     int status = 1;
     sp2_status.download(&status, 1, stream_);
+    have_factor = status == 0;
     return status == 0;
   };
   ScfOutcome out;

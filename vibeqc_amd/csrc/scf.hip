@@ -345,7 +345,13 @@ constexpr int kCholMaxLds = 8000;  // doubles of dynamic LDS (d[n], pivot row gp
 
 __global__ __launch_bounds__(kCholThreads) void pivoted_cholesky_kernel(const double* __restrict__ P, int n, int r,
                                                                         double* __restrict__ G,
-                                                                        int* __restrict__ status) {
+                                                                        int* __restrict__ status,
+                                                                        const int* __restrict__ done) {
+  // A factor already made from the previous one (done[0] == 0) stands.
+  if (done && done[0] == 0) {
+    if (threadIdx.x == 0) status[0] = 0;
+    return;
+  }
   extern __shared__ double dyn[];
   double* d = dyn;       // remaining diagonal
   double* gp = dyn + n;  // row `pivot` of G, columns < k
@@ -409,6 +415,42 @@ __global__ __launch_bounds__(kCholThreads) void pivoted_cholesky_kernel(const do
     __syncthreads();
   }
   if (tid == 0) status[0] = 0;
+}
+
+// Projected factor check: status[0] = 0 when the Cholesky of A^T A succeeded
+// (info 0) and its diagonal spans at most kProjectedSpread (the projected
+// previous factor lost no direction of the new projector), else 1; then
+// G <- A when accepted.
+constexpr double kProjectedSpread = 1e3;
+__global__ void projected_factor_check_kernel(const rocblas_int* __restrict__ info, const double* __restrict__ R, int r,
+                                              int* __restrict__ status) {
+  __shared__ double lo[256], hi[256];
+  double a = INFINITY, b = 0.0;
+  for (int i = threadIdx.x; i < r; i += blockDim.x) {
+    const double d = fabs(R[static_cast<std::size_t>(i) * r + i]);
+    a = fmin(a, d);
+    b = fmax(b, d);
+  }
+  lo[threadIdx.x] = a;
+  hi[threadIdx.x] = b;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      lo[threadIdx.x] = fmin(lo[threadIdx.x], lo[threadIdx.x + off]);
+      hi[threadIdx.x] = fmax(hi[threadIdx.x], hi[threadIdx.x + off]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    status[0] = info[0] == 0 && lo[0] > 0.0 && isfinite(hi[0]) && hi[0] <= kProjectedSpread * lo[0] ? 0 : 1;
+}
+
+__global__ void copy_if_accepted_kernel(double* __restrict__ dst, const double* __restrict__ src, std::size_t count,
+                                        const int* __restrict__ status) {
+  if (status[0] != 0) return;
+  const std::size_t stride = static_cast<std::size_t>(gridDim.x) * blockDim.x;
+  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride)
+    dst[i] = src[i];
 }
 
 __global__ void copy_kernel(double* __restrict__ dst, const double* __restrict__ src, std::size_t count) {
@@ -531,11 +573,29 @@ Sp2Summary sp2_summary(const double* h, int steps) {
 
 bool pivoted_cholesky_fits(int n, int r) { return r >= 1 && r <= n && n + r <= kCholMaxLds; }
 
-void pivoted_cholesky(const double* P, int n, int r, double* G, int* status, hipStream_t s) {
+void pivoted_cholesky(const double* P, int n, int r, double* G, int* status, hipStream_t s, const int* done) {
   if (!pivoted_cholesky_fits(n, r)) throw std::invalid_argument("pivoted Cholesky size out of range");
   const std::size_t lds = static_cast<std::size_t>(n + r) * sizeof(double);
-  hipLaunchKernelGGL(pivoted_cholesky_kernel, dim3(1), dim3(kCholThreads), lds, s, P, n, r, G, status);
+  hipLaunchKernelGGL(pivoted_cholesky_kernel, dim3(1), dim3(kCholThreads), lds, s, P, n, r, G, status, done);
   VQC_LAUNCH("pivoted_cholesky_kernel");
+}
+
+void projected_factor(rocblas_handle blas, const double* P, int n, int r, double* G, double* A, double* M,
+                      rocblas_int* info, int* status, hipStream_t s) {
+  const double one = 1.0, zero = 0.0;
+  // A = P G, M = A^T A = R^T R, A <- A R^-1.
+  VQC_BLAS(rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, n, r, n, &one, P, n, G, n, &zero, A, n));
+  VQC_BLAS(rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, r, r, n, &one, A, n, A, n, &zero, M,
+                         r));
+  VQC_BLAS(rocsolver_dpotrf(blas, rocblas_fill_upper, r, M, r, info));
+  VQC_BLAS(rocblas_dtrsm(blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                         rocblas_diagonal_non_unit, n, r, &one, M, r, A, n));
+  hipLaunchKernelGGL(projected_factor_check_kernel, dim3(1), dim3(256), 0, s, info, M, r, status);
+  VQC_LAUNCH("projected_factor_check_kernel");
+  const std::size_t count = static_cast<std::size_t>(n) * r;
+  hipLaunchKernelGGL(copy_if_accepted_kernel, dim3(std::min<unsigned>(blocks_for(count), 4096u)), dim3(256), 0, s, G,
+                     A, count, status);
+  VQC_LAUNCH("copy_if_accepted_kernel");
 }
 
 void note_solver_failure(const rocblas_int* info, double* flag, hipStream_t s) {

@@ -75,6 +75,16 @@ Sp2Summary sp2_summary(const double* host_state, int steps);
 // when a pivot falls below 1e-8, i.e. P is not of rank r.  Requires
 // pivoted_cholesky_fits(n, r).
 bool pivoted_cholesky_fits(int n, int r);
-void pivoted_cholesky(const double* P, int n, int r, double* G, int* status, hipStream_t s);
+// With `done` non-null and done[0] == 0 (a factor was already accepted by
+// projected_factor), status[0] = 0 and G is left as it is.
+void pivoted_cholesky(const double* P, int n, int r, double* G, int* status, hipStream_t s, const int* done = nullptr);
+// Factor of a projector P (rank r) from the previous iteration's factor G
+// (n x r, column-major), when its projection still spans P's range:
+// A = P G, A^T A = R^T R, G <- A R^-1 (so G G^T = P).  status[0] = 0 when
+// accepted (Cholesky info 0 and R's diagonal within a factor 1e3), else 1 and
+// G is unchanged.  A (n x r) and M (r x r) are workspaces; everything stays on
+// the stream.
+void projected_factor(rocblas_handle blas, const double* P, int n, int r, double* G, double* A, double* M,
+                      rocblas_int* info, int* status, hipStream_t s);
 
 }  // namespace vqc

@@ -210,7 +210,8 @@ ScfOutcome DfPlan::solve_unrestricted(const ScfOptions& o, const std::vector<dou
     gemm(rocblas_operation_none, rocblas_operation_none, n, X.data(), n, V.data(), n, W.data() + s * nn);
   };
   // Occupied block of F_s during the iterations: SP2 purification of
-  // F~ = X F_s X and a pivoted Cholesky factor G of the projector (the first
+  // F~ = X F_s X and a factor G of the projector (projected from the previous
+  // iteration's, else a pivoted Cholesky factor; the first
   // n_s columns of W_s are X G), as in the restricted solve; a spin whose
   // purification does not reach an integer trace, or whose factor is rank
   // deficient, is diagonalised instead.  The core guess (its frontier
@@ -219,9 +220,14 @@ ScfOutcome DfPlan::solve_unrestricted(const ScfOptions& o, const std::vector<dou
   const int omax = std::max(na, nb);
   const bool sp2_spin[2] = {na > 0 && na < n && pivoted_cholesky_fits(n, na),
                             nb > 0 && nb < n && pivoted_cholesky_fits(n, nb)};
-  DeviceArray<double> G(static_cast<std::size_t>(n) * std::max(omax, 1)), sp2_state(kSp2StateSize),
-      sp2_partial(sp2_partial_size(n));
-  DeviceArray<int> sp2_status(1);
+  // One projector factor per spin (n x n_s at offset s n omax), kept between
+  // iterations: each new one is first tried as a projection of the previous.
+  const std::size_t gstride = static_cast<std::size_t>(n) * std::max(omax, 1);
+  DeviceArray<double> G(2 * gstride), sp2_state(kSp2StateSize), sp2_partial(sp2_partial_size(n));
+  DeviceArray<double> projA(gstride), projM(static_cast<std::size_t>(std::max(omax, 1)) * std::max(omax, 1));
+  DeviceArray<int> sp2_status(1), proj_status(1);
+  DeviceArray<rocblas_int> proj_info(1);
+  bool have_factor[2] = {false, false};
   int sp2_hint[2] = {0, 0};
   auto purify = [&](int s) {  // T2 -> projector in V; T1 is scratch
     const double target = static_cast<double>(occ[s]);
@@ -248,12 +254,18 @@ ScfOutcome DfPlan::solve_unrestricted(const ScfOptions& o, const std::vector<dou
       gemm(rocblas_operation_none, rocblas_operation_none, n, Fs, n, X.data(), n, T1.data());
       gemm(rocblas_operation_none, rocblas_operation_none, n, X.data(), n, T1.data(), n, T2.data());
       if (purify(s)) {
-        pivoted_cholesky(V.data(), n, occ[s], G.data(), sp2_status.data(), stream_);
+        double* Gs = G.data() + s * gstride;
+        if (have_factor[s])
+          projected_factor(blas_, V.data(), n, occ[s], Gs, projA.data(), projM.data(), proj_info.data(),
+                           proj_status.data(), stream_);
+        pivoted_cholesky(V.data(), n, occ[s], Gs, sp2_status.data(), stream_,
+                         have_factor[s] ? proj_status.data() : nullptr);
         int status = 1;
         sp2_status.download(&status, 1, stream_);
+        have_factor[s] = status == 0;
         if (status == 0) {
           VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, n, occ[s], n, &one, X.data(),
-                                 n, G.data(), n, &zero, W.data() + s * nn, n));
+                                 n, Gs, n, &zero, W.data() + s * nn, n));
           ++out.purified_iterations;
           return;
         }
