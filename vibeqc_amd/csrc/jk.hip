@@ -72,6 +72,14 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
   lo = static_cast<__bf16>(x - static_cast<float>(hi));
 }
 
+// LDS tiles of 32 bf16 per row (64 bytes, four 16-byte chunks, no padding)
+// that MFMA fragments read as 16 consecutive rows at one chunk: chunk c of
+// row r is stored at position c ^ ((r >> 1) & 3), so the 16 reads of a
+// fragment cover all 64 banks once (a padded 80-byte row leaves 2-way
+// conflicts; measured 3.4e9 conflict cycles per (H2O)64 U launch, zero with
+// this layout).
+__device__ __forceinline__ int lds_chunk(int row, int chunk) { return chunk ^ ((row >> 1) & 3); }
+
 // Split of eight floats into bf16 hi/lo fragments, two at a time (one packed
 // conversion per pair for hi and one for lo).
 __device__ __forceinline__ void split8_bf16(const float (&x)[8], bf8& hi, bf8& lo) {
@@ -358,7 +366,7 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
   // 16 waves: WQ along Q x (16 / WQ) along i.  WQ = 4 gives 64 x 8*NIW wave
   // tiles (even NIW), which read fewer LDS fragments per MFMA than WQ = 8
   // (32 x 16*NIW).
-  constexpr int BQ = kUTileQ, BK = 32, BI = 32 * NIW, PAD = 8, NT = 1024;
+  constexpr int BQ = kUTileQ, BK = 32, BI = 32 * NIW, NT = 1024;
   constexpr int WI = 16 / WQ, XF = BQ / (16 * WQ), YF = BI / (16 * WI);
   static_assert(YF * 16 * WI == BI, "i tile must split evenly over the waves");
   constexpr int YP = (BI * 4 + NT - 1) / NT;  // 16-byte C chunks per thread and plane
@@ -366,8 +374,9 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
   // into the other LDS buffer while the rest are still multiplying, with one
   // barrier per step.  NBUF = 1 (wide i tiles, whose two buffers would not
   // fit beside the rho row): store after a barrier, then a second barrier.
-  __shared__ __bf16 Xh[NBUF][BQ][BK + PAD], Xl[NBUF][BQ][BK + PAD];
-  __shared__ __bf16 Yh[NBUF][BI][BK + PAD], Yl[NBUF][BI][BK + PAD];
+  // Rows of 64 bytes with swizzled chunks (lds_chunk).
+  __shared__ __bf16 Xh[NBUF][BQ][BK], Xl[NBUF][BQ][BK];
+  __shared__ __bf16 Yh[NBUF][BI][BK], Yl[NBUF][BI][BK];
   // Q-tile-major order: the blocks in flight share one Q tile, so the
   // second gather of each packed row (as (mu, nu) and as (nu, mu)) finds it
   // in the last-level cache instead of HBM.
@@ -443,15 +452,16 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
       h[e] = hh;
       l[e] = ll;
     }
-    *reinterpret_cast<bf8*>(&Xh[buf][xq][xg * 8]) = h;
-    *reinterpret_cast<bf8*>(&Xl[buf][xq][xg * 8]) = l;
+    *reinterpret_cast<bf8*>(&Xh[buf][xq][lds_chunk(xq, xg) * 8]) = h;
+    *reinterpret_cast<bf8*>(&Xl[buf][xq][lds_chunk(xq, xg) * 8]) = l;
 #pragma unroll
     for (int j = 0; j < YP; ++j) {
       const int e = tid + j * NT;
       if (e < BI * 4) {
         const std::size_t off = static_cast<std::size_t>(i0 + (e >> 2)) * ldct + nu0 + (e & 3) * 8;
-        *reinterpret_cast<bf8*>(&Yh[buf][e >> 2][(e & 3) * 8]) = *reinterpret_cast<const bf8*>(Cth + off);
-        *reinterpret_cast<bf8*>(&Yl[buf][e >> 2][(e & 3) * 8]) = *reinterpret_cast<const bf8*>(Ctl + off);
+        const int yc = lds_chunk(e >> 2, e & 3) * 8;
+        *reinterpret_cast<bf8*>(&Yh[buf][e >> 2][yc]) = *reinterpret_cast<const bf8*>(Cth + off);
+        *reinterpret_cast<bf8*>(&Yl[buf][e >> 2][yc]) = *reinterpret_cast<const bf8*>(Ctl + off);
       }
     }
   };
@@ -461,7 +471,9 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
   __syncthreads();
   int cur = 0;
   for (int nu0 = 0; nu0 < n; nu0 += BK) {
-    const int kq = 8 * (lane >> 4);
+    // Fragment rows start at multiples of 16, so the swizzled chunk depends
+    // on the lane only.
+    const int kq = lds_chunk(lane & 15, lane >> 4) * 8;
     bf8 ah[XF], al[XF];
 #pragma unroll
     for (int x = 0; x < XF; ++x) {
@@ -499,7 +511,8 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
     }
   }
   if (with_rho) {
-    double* red = reinterpret_cast<double*>(&Xh[0][0][0]);  // NT doubles fit in Xh
+    static_assert(sizeof(Xh) >= NT * sizeof(double), "the rho reduction reuses Xh");
+    double* red = reinterpret_cast<double*>(&Xh[0][0][0]);
     red[xg * BQ + xq] = racc;
     __syncthreads();
     if (tid < BQ && q0 + tid < ldq) {
@@ -537,10 +550,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NIW <= 3 ? 
     const float* __restrict__ B, const __bf16* __restrict__ Cth, const __bf16* __restrict__ Ctl,
     __bf16* __restrict__ Uh, __bf16* __restrict__ Ul, std::size_t us, int n, int ldq, int ldct, int r,
     const double* __restrict__ D, double* __restrict__ rho_part) {
-  constexpr int NW = 8, NT = 64 * NW, BQ = 32 * NW, BK = 32, BI = 32 * NIW, PAD = 8;
+  constexpr int NW = 8, NT = 64 * NW, BQ = 32 * NW, BK = 32, BI = 32 * NIW;
   constexpr int XF = 2, YF = BI / 16;
   constexpr int YP = (BI * 4 + NT - 1) / NT;  // 16-byte C^T chunks per thread and plane
-  __shared__ __bf16 Yh[2][BI][BK + PAD], Yl[2][BI][BK + PAD];
+  // C^T rows of 64 bytes, chunks swizzled (lds_chunk): the sixteen rows one
+  // fragment read touches hit distinct banks.
+  __shared__ __bf16 Yh[2][BI][BK], Yl[2][BI][BK];
   // Dynamic LDS: element offsets of the packed rows (mu, nu) for nu < n + BK
   // (rows past n repeat row n - 1, so gathers need no bounds test), then, for
   // the rho pass, D[mu][nu] as float.
@@ -589,8 +604,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NIW <= 3 ? 
     for (int jj = 0; jj < YP; ++jj) {
       const int e = tid + jj * NT;
       if (e < BI * 4) {
-        *reinterpret_cast<bf8*>(&Yh[buf][e >> 2][(e & 3) * 8]) = rch[jj];
-        *reinterpret_cast<bf8*>(&Yl[buf][e >> 2][(e & 3) * 8]) = rcl[jj];
+        const int yc = lds_chunk(e >> 2, e & 3) * 8;
+        *reinterpret_cast<bf8*>(&Yh[buf][e >> 2][yc]) = rch[jj];
+        *reinterpret_cast<bf8*>(&Yl[buf][e >> 2][yc]) = rcl[jj];
       }
     }
   };
@@ -637,7 +653,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NIW <= 3 ? 
       cload(nu0 + BK);
       if (q_ok) gload(nu0 + BK);
     }
-    const int kq = 8 * kg;
+    const int kq = lds_chunk(lane & 15, kg) * 8;  // fragment rows start at multiples of 16
     if (q_ok) {
 #pragma unroll
       for (int y = 0; y < YF; ++y) {
@@ -735,9 +751,10 @@ __global__ __launch_bounds__(512) void exchange_syrk_bf16x3_kernel(const __bf16*
                                                                   std::size_t kbegin, std::size_t kend,
                                                                   std::size_t kchunk, double sign,
                                                                   float* __restrict__ partial) {
-  constexpr int T = kSyrkBfT, BK = 32, PAD = 8, NT = 512, XF = 8, YF = 4;
-  __shared__ __bf16 Ah[T][BK + PAD], Al[T][BK + PAD];
-  __shared__ __bf16 Bh[T][BK + PAD], Bl[T][BK + PAD];
+  constexpr int T = kSyrkBfT, BK = 32, NT = 512, XF = 8, YF = 4;
+  // Rows of 64 bytes with swizzled chunks (lds_chunk).
+  __shared__ __bf16 Ah[T][BK], Al[T][BK];
+  __shared__ __bf16 Bh[T][BK], Bl[T][BK];
   const int per = gridDim.x / 8;
   const int work = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (work >= nwork) return;
@@ -780,7 +797,7 @@ __global__ __launch_bounds__(512) void exchange_syrk_bf16x3_kernel(const __bf16*
   auto sstore = [&]() {
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
-      const int c = tid + j * NT, row = c >> 2, kc = (c & 3) << 3;
+      const int c = tid + j * NT, row = c >> 2, kc = lds_chunk(row, c & 3) << 3;
       *reinterpret_cast<bf8*>(&Ah[row][kc]) = ra[j][0];
       *reinterpret_cast<bf8*>(&Al[row][kc]) = ra[j][1];
       if (!diag) {
@@ -795,7 +812,7 @@ __global__ __launch_bounds__(512) void exchange_syrk_bf16x3_kernel(const __bf16*
   for (std::size_t kb = k0; kb < k1; kb += BK) {
     const bool more = kb + BK < k1;
     if (more) gload(kb + BK);
-    const int kq = 8 * (lane >> 4);
+    const int kq = lds_chunk(lane & 15, lane >> 4) * 8;  // fragment rows start at multiples of 16
     bf8 bh[YF], bl[YF];
 #pragma unroll
     for (int j = 0; j < YF; ++j) {
