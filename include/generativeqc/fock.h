@@ -235,6 +235,49 @@ GENERATIVEQC_API generativeqc_status vibeqc_fock_plan_streaming(const generative
 GENERATIVEQC_API generativeqc_status vibeqc_fock_plan_last_solve_stats(const generativeqc_fock_plan* plan,
                                                                        uint64_t* stats3);
 
+/* Extension (no counterpart in the reference's API; its schedules are tuned
+ * at kernel generation time and reused per device, generativeqc.h:1191-1212):
+ * measured launch shapes for the occupied-factor J/K build.  Times the build
+ * of C (row-major nbf x nocc, as vibeqc_fock_plan_benchmark_occupied) at the
+ * plan's precision for the default shape and for alternative U i-tile widths
+ * (32-column units; the occupied columns in one, two or three i blocks) and
+ * SYRK split-K counts (half and twice the default), `repeat` builds each, and
+ * records the fastest for this device architecture, precision, nbf, nocc and
+ * padded auxiliary count in the process-wide registry that later builds of
+ * that shape (on any plan) use.  shape2 (may be NULL) receives the chosen
+ * [U tile, SYRK splits]; ms2 (may be NULL) [best, default] average build ms.
+ * A different shape only regroups sums: J is unchanged and K changes at
+ * rounding level.  Resident plans only (NOT_IMPLEMENTED when streamed). */
+GENERATIVEQC_API generativeqc_status vibeqc_fock_plan_autotune(generativeqc_fock_plan* plan, const double* occupied,
+                                                               uint64_t nbf, uint64_t nocc, uint32_t repeat,
+                                                               int32_t* shape2, double* ms2);
+
+/* The [U tile, SYRK splits] that builds of nocc occupied columns use at the
+ * plan's current precision (recorded shape, else the default rules). */
+GENERATIVEQC_API generativeqc_status vibeqc_fock_plan_launch_shape(const generativeqc_fock_plan* plan, uint64_t nocc,
+                                                                   int32_t* shape2);
+
+/* The launch-shape registry itself (host only, no device needed).  Keys are
+ * (arch, precision 0 FP64 / 1 FP32 / 2 BF16X3, nbf, occupied columns, ldq =
+ * auxiliary count rounded up to 128).  u_tile 0..10 and syrk_splits 0..128,
+ * 0 meaning the default rule.  With VIBEQC_TUNING_FILE set, the registry
+ * reads that file on first use and rewrites it after every record.  Files
+ * are text: the line "vibeqc-jk-tuning 1", then one
+ * "<arch> <precision> <nbf> <occupied> <ldq> <u_tile> <syrk_splits> <ms>"
+ * line per entry.  load returns *found = 0 for a missing file and
+ * INVALID_ARGUMENT for a malformed one; record and save return
+ * INTERNAL_ERROR when a file cannot be written. */
+GENERATIVEQC_API generativeqc_status vibeqc_tuning_record(const char* arch, int32_t precision, uint64_t nbf,
+                                                          uint64_t occupied, uint64_t ldq, int32_t u_tile,
+                                                          int32_t syrk_splits, double ms);
+GENERATIVEQC_API generativeqc_status vibeqc_tuning_lookup(const char* arch, int32_t precision, uint64_t nbf,
+                                                          uint64_t occupied, uint64_t ldq, int32_t* shape2,
+                                                          int32_t* found);
+GENERATIVEQC_API generativeqc_status vibeqc_tuning_count(uint64_t* count);
+GENERATIVEQC_API generativeqc_status vibeqc_tuning_load(const char* path, int32_t* found);
+GENERATIVEQC_API generativeqc_status vibeqc_tuning_save(const char* path);
+GENERATIVEQC_API generativeqc_status vibeqc_tuning_clear(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -342,6 +342,23 @@ EXPORTS = {
     "vibeqc_fock_plan_shape": (ct.c_int32, [ct.c_void_p, ct.POINTER(ct.c_uint64)]),
     "vibeqc_fock_plan_streaming": (ct.c_int32, [ct.c_void_p, ct.POINTER(ct.c_uint64)]),
     "vibeqc_fock_plan_last_solve_stats": (ct.c_int32, [ct.c_void_p, ct.POINTER(ct.c_uint64)]),
+    "vibeqc_fock_plan_autotune": (
+        ct.c_int32,
+        [ct.c_void_p, _DP, ct.c_uint64, ct.c_uint64, ct.c_uint32, ct.POINTER(ct.c_int32), _DP],
+    ),
+    "vibeqc_fock_plan_launch_shape": (ct.c_int32, [ct.c_void_p, ct.c_uint64, ct.POINTER(ct.c_int32)]),
+    "vibeqc_tuning_record": (
+        ct.c_int32,
+        [ct.c_char_p, ct.c_int32, ct.c_uint64, ct.c_uint64, ct.c_uint64, ct.c_int32, ct.c_int32, ct.c_double],
+    ),
+    "vibeqc_tuning_lookup": (
+        ct.c_int32,
+        [ct.c_char_p, ct.c_int32, ct.c_uint64, ct.c_uint64, ct.c_uint64, ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int32)],
+    ),
+    "vibeqc_tuning_count": (ct.c_int32, [ct.POINTER(ct.c_uint64)]),
+    "vibeqc_tuning_load": (ct.c_int32, [ct.c_char_p, ct.POINTER(ct.c_int32)]),
+    "vibeqc_tuning_save": (ct.c_int32, [ct.c_char_p]),
+    "vibeqc_tuning_clear": (ct.c_int32, []),
     "vibeqc_fock_plan_one_electron": (ct.c_int32, [ct.c_void_p, _DP, _DP, ct.c_uint64]),
     "generativeqc_batch_prepare": (
         ct.c_int32,

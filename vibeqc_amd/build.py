@@ -27,7 +27,7 @@ GRAD_UNITS = tuple(f"integrals_g{a}{b}.hip" for a in range(4) for b in range(a +
 SOURCES = (
     tuple(reversed(GRAD_UNITS))
     + tuple(reversed(CLASS_UNITS))
-    + ("integrals.hip", "integrals_grad.hip", "jk.hip", "noalloc.hip", "plan.hip", "uhf.hip", "gradient.hip", "scf.hip", "basis.cpp", "batch.cpp", "capi.cpp")
+    + ("integrals.hip", "integrals_grad.hip", "jk.hip", "noalloc.hip", "plan.hip", "uhf.hip", "gradient.hip", "scf.hip", "basis.cpp", "batch.cpp", "capi.cpp", "tuning.cpp")
 )
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 

@@ -24,6 +24,7 @@
 #include "generativeqc/generativeqc.h"
 #include "noalloc.hpp"
 #include "plan.hpp"
+#include "tuning.hpp"
 
 struct generativeqc_context {
   int device = 0;
@@ -659,6 +660,126 @@ generativeqc_status vibeqc_fock_plan_streaming(const generativeqc_fock_plan* pla
 generativeqc_status vibeqc_fock_plan_last_solve_stats(const generativeqc_fock_plan* plan, uint64_t* stats3) {
   if (!plan || !stats3) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
   std::copy(plan->solve_stats, plan->solve_stats + 3, stats3);
+  return GENERATIVEQC_STATUS_SUCCESS;
+}
+
+generativeqc_status vibeqc_fock_plan_autotune(generativeqc_fock_plan* plan, const double* occupied, uint64_t nbf,
+                                              uint64_t nocc, uint32_t repeat, int32_t* shape2, double* ms2) {
+  if (!plan || !occupied) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  plan->detail.clear();
+  try {
+    auto& p = *plan->plan;
+    require(nbf == static_cast<uint64_t>(p.nbf()) && nocc > 0 && nocc <= nbf, "occupied factor shape mismatch");
+    const std::vector<double> c(occupied, occupied + nbf * nocc);
+    const vqc::AutotuneResult r = p.autotune(c, static_cast<int>(nocc), repeat);
+    if (shape2) {
+      shape2[0] = r.u_tile;
+      shape2[1] = r.syrk_splits;
+    }
+    if (ms2) {
+      ms2[0] = r.best_ms;
+      ms2[1] = r.default_ms;
+    }
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(&plan->detail);
+  }
+}
+
+generativeqc_status vibeqc_fock_plan_launch_shape(const generativeqc_fock_plan* plan, uint64_t nocc, int32_t* shape2) {
+  if (!plan || !shape2 || nocc == 0) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  try {
+    const auto& p = *plan->plan;
+    if (nocc > static_cast<uint64_t>(p.nbf())) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+    const int r = static_cast<int>(nocc);
+    const vqc::JkShape s = p.shape_for(p.precision(), r);
+    shape2[0] = s.u_tile > 0 ? s.u_tile : vqc::occupied_tile(r);
+    shape2[1] = s.syrk_splits > 0 ? s.syrk_splits : vqc::exchange_default_splits(p.precision(), p.nbf());
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(nullptr);
+  }
+}
+
+generativeqc_status vibeqc_tuning_record(const char* arch, int32_t precision, uint64_t nbf, uint64_t occupied,
+                                         uint64_t ldq, int32_t u_tile, int32_t syrk_splits, double ms) {
+  if (!arch || nbf > static_cast<uint64_t>(std::numeric_limits<int>::max()) || occupied > nbf ||
+      ldq > static_cast<uint64_t>(std::numeric_limits<int>::max()) || !std::isfinite(ms))
+    return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  try {
+    vqc::TuningEntry e;
+    e.arch = arch;
+    e.precision = precision;
+    e.n = static_cast<int>(nbf);
+    e.r = static_cast<int>(occupied);
+    e.ldq = static_cast<int>(ldq);
+    e.shape = vqc::JkShape{u_tile, syrk_splits};
+    e.ms = ms;
+    vqc::record_shape(e);
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (const std::invalid_argument&) {
+    return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  } catch (const std::runtime_error&) {
+    return GENERATIVEQC_STATUS_INTERNAL_ERROR;  // VIBEQC_TUNING_FILE could not be written
+  } catch (...) {
+    return map_exception(nullptr);
+  }
+}
+
+generativeqc_status vibeqc_tuning_lookup(const char* arch, int32_t precision, uint64_t nbf, uint64_t occupied,
+                                         uint64_t ldq, int32_t* shape2, int32_t* found) {
+  if (!arch || !shape2 || !found || nbf > static_cast<uint64_t>(std::numeric_limits<int>::max()) ||
+      occupied > nbf || ldq > static_cast<uint64_t>(std::numeric_limits<int>::max()))
+    return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  try {
+    vqc::JkShape s;
+    *found = vqc::tuned_shape(arch, precision, static_cast<int>(nbf), static_cast<int>(occupied), static_cast<int>(ldq), s)
+                 ? 1
+                 : 0;
+    shape2[0] = *found ? s.u_tile : 0;
+    shape2[1] = *found ? s.syrk_splits : 0;
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(nullptr);
+  }
+}
+
+generativeqc_status vibeqc_tuning_count(uint64_t* count) {
+  if (!count) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  try {
+    *count = vqc::tuned_shapes().size();
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(nullptr);
+  }
+}
+
+generativeqc_status vibeqc_tuning_load(const char* path, int32_t* found) {
+  if (!path || !found) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  try {
+    *found = vqc::load_tuning(path) ? 1 : 0;
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (...) {
+    return map_exception(nullptr);
+  }
+}
+
+generativeqc_status vibeqc_tuning_save(const char* path) {
+  if (!path) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  try {
+    vqc::save_tuning(path);
+    return GENERATIVEQC_STATUS_SUCCESS;
+  } catch (const std::invalid_argument&) {
+    return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  } catch (const std::runtime_error&) {
+    return GENERATIVEQC_STATUS_INTERNAL_ERROR;  // the file could not be written
+  } catch (...) {
+    return map_exception(nullptr);
+  }
+}
+
+generativeqc_status vibeqc_tuning_clear(void) {
+  vqc::clear_tuning();
   return GENERATIVEQC_STATUS_SUCCESS;
 }
 

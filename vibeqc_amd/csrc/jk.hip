@@ -1169,16 +1169,37 @@ std::size_t u_row_stride(Precision prec, int r, int ldq) {
   return dense + (prec == Precision::FP32 ? 32 : 64);
 }
 
+namespace {
+// Slices per run: both runs (positive and negative columns) together must
+// fit the partial buffer (n^2 doubles per FP64 slice; FP32 slices use half).
+int syrk_cap(int n, std::size_t partial_capacity) {
+  const std::size_t slice = static_cast<std::size_t>(n) * n;
+  return partial_capacity == 0
+             ? syrk_split_cap(n)
+             : static_cast<int>(std::min<std::size_t>(syrk_split_cap(n), partial_capacity / (2 * slice)));
+}
+int syrk_tile_count(Precision prec, int n, int& ncb) {
+  return prec == Precision::BF16X3 ? syrk_tiles(n, kSyrkBfT, kSyrkBfT, ncb) : syrk_tiles(n, kSyrkTM, kSyrkTN, ncb);
+}
+// Resident 512-thread blocks on 256 CUs: one per CU for BF16X3 (64 KB LDS)
+// and FP64 (102 KB), two for FP32 (51 KB).
+int syrk_slots(Precision prec) { return prec == Precision::FP32 ? 512 : 256; }
+}  // namespace
+
+int exchange_max_splits(int n, std::size_t partial_capacity) { return std::max(1, syrk_cap(n, partial_capacity)); }
+
+int exchange_default_splits(Precision prec, int n, std::size_t partial_capacity) {
+  const int cap = syrk_cap(n, partial_capacity);
+  if (cap < 1) throw std::length_error("exchange partial buffer holds no slice");
+  int ncb = 0;
+  return syrk_splits(syrk_tile_count(prec, n, ncb), cap, syrk_slots(prec));
+}
+
 void exchange_syrk(Precision prec, const void* U, std::size_t ustride, int n, int r, int r_pos, int ldq, double* K,
-                   double* partial, hipStream_t s, std::size_t partial_capacity) {
+                   double* partial, hipStream_t s, std::size_t partial_capacity, int split_override) {
   const std::size_t kd = static_cast<std::size_t>(r) * ldq;
   if (ustride < kd) throw std::invalid_argument("U row stride shorter than r x ldq");
-  // Slices per run: both runs (positive and negative columns) together must
-  // fit the partial buffer (n^2 doubles per FP64 slice; FP32 slices use half).
-  const std::size_t slice = static_cast<std::size_t>(n) * n;
-  const int cap = partial_capacity == 0
-                      ? syrk_split_cap(n)
-                      : static_cast<int>(std::min<std::size_t>(syrk_split_cap(n), partial_capacity / (2 * slice)));
+  const int cap = syrk_cap(n, partial_capacity);
   if (cap < 1) throw std::length_error("exchange partial buffer holds no slice");
   int slices = 0;
   auto run = [&](std::size_t kb, std::size_t ke, double sign) {
@@ -1189,13 +1210,9 @@ void exchange_syrk(Precision prec, const void* U, std::size_t ustride, int n, in
     double* out64 = partial + at;
     float* out32 = reinterpret_cast<float*>(partial) + at;
     int ncb = 0;
-    const bool bf = prec == Precision::BF16X3;
-    const int tiles = bf ? syrk_tiles(n, kSyrkBfT, kSyrkBfT, ncb) : syrk_tiles(n, kSyrkTM, kSyrkTN, ncb);
-    const std::size_t bk = bf ? 32 : 16;
-    // Resident 512-thread blocks on 256 CUs: one per CU for BF16X3 (80 KB LDS)
-    // and FP64 (102 KB), two for FP32 (51 KB).
-    const int slots = prec == Precision::FP32 ? 512 : 256;
-    int splits = syrk_splits(tiles, cap, slots);
+    const int tiles = syrk_tile_count(prec, n, ncb);
+    const std::size_t bk = prec == Precision::BF16X3 ? 32 : 16;
+    int splits = split_override > 0 ? std::min(split_override, cap) : syrk_splits(tiles, cap, syrk_slots(prec));
     const std::size_t chunk = round_up((len + splits - 1) / splits, bk);
     splits = static_cast<int>((len + chunk - 1) / chunk);
     const int nwork = tiles * splits;

@@ -63,10 +63,17 @@ void occupied_u(Precision prec, const void* B, const OccupiedOperands& ops, int 
 // W[mu] = U[mu] (r ldq elements, rows ustride apart) and columns i < r_pos
 // carry +1, the rest -1.  `partial` needs exchange_partial_size(n) doubles.
 // `partial_capacity` (doubles) bounds the split-K slices to what `partial`
-// holds (0: exchange_partial_size(n)).
+// holds (0: exchange_partial_size(n)).  `splits` > 0 sets the split-K slices
+// per run (at most what the partial buffer holds) in place of the occupancy
+// rule; the slices only regroup the k sum, so K changes at rounding level.
 std::size_t exchange_partial_size(int n);
 void exchange_syrk(Precision prec, const void* U, std::size_t ustride, int n, int r, int r_pos, int ldq, double* K,
-                   double* partial, hipStream_t s, std::size_t partial_capacity = 0);
+                   double* partial, hipStream_t s, std::size_t partial_capacity = 0, int splits = 0);
+// The split-K slices per run that the occupancy rule picks for this precision
+// and n (whole waves of resident blocks on 256 CUs), under the same cap.
+int exchange_default_splits(Precision prec, int n, std::size_t partial_capacity = 0);
+// The most split-K slices per run the partial buffer holds (at most 128).
+int exchange_max_splits(int n, std::size_t partial_capacity = 0);
 // C (row-major n x nocc) -> factor n x ldc scaled by sqrt(2), zero padded, and
 // D = (that factor)(that factor)^T = 2 C C^T (n x n); both fully written.
 void occupied_factor_and_density(const double* C, int n, int nocc, int ldc, double* factor, double* D,

@@ -22,6 +22,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 
@@ -94,10 +96,30 @@ void check_basis_limits(const System& sys, int lmax, const char* what) {
 
 }  // namespace
 
+namespace {
+// gfx architecture name of a device (cached: plans of a batch are created by
+// the hundred).
+std::string device_arch(int device) {
+  static std::mutex mutex;
+  static std::map<int, std::string> names;
+  std::lock_guard<std::mutex> lock(mutex);
+  auto it = names.find(device);
+  if (it != names.end()) return it->second;
+  hipDeviceProp_t prop{};
+  VQC_HIP(hipGetDeviceProperties(&prop, device));
+  std::string name(prop.gcnArchName);
+  name = name.substr(0, name.find(':'));
+  if (name.empty()) name = "unknown";
+  names[device] = name;
+  return name;
+}
+}  // namespace
+
 DfPlan::DfPlan(int device, const System& orbital, const System& auxiliary, const PlanOptions& options,
                hipStream_t stream, rocblas_handle blas)
     : device_(device), orb_(orbital), opts_(options) {
   VQC_HIP(hipSetDevice(device_));
+  arch_ = device_arch(device_);
   if (stream && blas) {
     stream_ = stream;
     blas_ = blas;
@@ -375,7 +397,8 @@ void DfPlan::ensure_factor(int r) {
     syrk_partial_.resize(streamed_ ? std::min(exchange_partial_size(n_), 10 * nn) : exchange_partial_size(n_));
   }
   r = std::max(r, 1);
-  const int ni = occupied_tile(r);
+  const JkShape shape = shape_for(prec_, r);
+  const int ni = shape.u_tile > 0 ? shape.u_tile : occupied_tile(r);
   const int ldc = static_cast<int>(round_up(r, 32 * ni));
   if (r <= factor_cap_ && ni == ni_ && ldc == ldc_) return;
   ni_ = ni;
@@ -416,7 +439,8 @@ void DfPlan::build_jk(Precision p, const double* D, int r, int r_pos) {
       prepare_occupied(p, ops_, stream_);
       occupied_u(p, B, ops_, ldq_, U_.data(), D, fused_rho ? rho_partial_.data() : nullptr, stream_);
     }
-    exchange_syrk(p, U_.data(), ops_.ustride, n_, r, r_pos, ldq_, K_.data(), syrk_partial_.data(), stream_);
+    exchange_syrk(p, U_.data(), ops_.ustride, n_, r, r_pos, ldq_, K_.data(), syrk_partial_.data(), stream_, 0,
+                  shape_for(p, std::max(r, 1)).syrk_splits);
   }
   if (opts_.coulomb) {
     if (fused_rho) {
@@ -874,7 +898,8 @@ std::array<double, 5> DfPlan::benchmark_occupied(const std::vector<double>& occu
     prepare_occupied(p, ops_, stream_);
     occupied_u(p, B, ops_, ldq_, U_.data(), D_.data(), rho_partial_.data(), stream_);
     VQC_HIP(hipEventRecord(ev[it * 4 + 1].e, stream_));
-    exchange_syrk(p, U_.data(), ops_.ustride, n_, nocc, nocc, ldq_, K_.data(), syrk_partial_.data(), stream_);
+    exchange_syrk(p, U_.data(), ops_.ustride, n_, nocc, nocc, ldq_, K_.data(), syrk_partial_.data(), stream_, 0,
+                  shape_for(p, nocc).syrk_splits);
     VQC_HIP(hipEventRecord(ev[it * 4 + 2].e, stream_));
     rho_from_partials(rho_partial_.data(), n_, ldq_, rho_.data(), stream_);
     coulomb_matrix(p, B, rho_.data(), npair_, ldq_, n_, J_.data(), stream_);
@@ -897,6 +922,75 @@ std::array<double, 5> DfPlan::benchmark_occupied(const std::vector<double>& occu
   ms[4] = wall;
   for (auto& x : ms) x /= repeat;
   return ms;
+}
+
+JkShape DfPlan::shape_for(Precision p, int r) const {
+  if (forcing_) return forced_;
+  JkShape shape;
+  if (!tuned_shape(arch_, static_cast<int>(p), n_, r, ldq_, shape)) return JkShape{};
+  return shape;
+}
+
+AutotuneResult DfPlan::autotune(const std::vector<double>& occupied, int nocc, unsigned repeat) {
+  VQC_HIP(hipSetDevice(device_));
+  if (repeat == 0) throw std::invalid_argument("repeat must be positive");
+  require_resident("autotuning");
+  if (nocc <= 0 || occupied.size() != static_cast<std::size_t>(n_) * nocc)
+    throw std::invalid_argument("occupied factor shape mismatch");
+  const Precision p = prec_;
+  // Candidates: the rule's shape first, then U tiles covering the occupied
+  // columns in one, two or three i blocks and SYRK split counts of half and
+  // twice the rule's (at most what the partial buffer holds).
+  const int blocks = (nocc + 31) / 32;
+  std::vector<int> tiles{0};
+  for (int k = 1; k <= 3; ++k) {
+    const int t = std::min(10, (blocks + k - 1) / k);
+    if (t >= 1 && std::find(tiles.begin(), tiles.end(), t) == tiles.end() && t != occupied_tile(nocc)) tiles.push_back(t);
+  }
+  const int rule_splits = exchange_default_splits(p, n_);
+  const int max_splits = exchange_max_splits(n_);
+  std::vector<int> splits{0};
+  for (int sp : {rule_splits / 2, rule_splits * 2}) {
+    sp = std::min(sp, max_splits);
+    if (sp >= 1 && sp != rule_splits && std::find(splits.begin(), splits.end(), sp) == splits.end())
+      splits.push_back(sp);
+  }
+  AutotuneResult out;
+  double best = std::numeric_limits<double>::infinity();
+  struct Reset {
+    DfPlan* plan;
+    ~Reset() {
+      plan->forcing_ = false;
+      plan->forced_ = JkShape{};
+    }
+  } reset{this};
+  for (int t : tiles)
+    for (int sp : splits) {
+      forcing_ = true;
+      forced_ = JkShape{t, sp};
+      const double ms = benchmark_occupied(occupied, nocc, repeat)[3];
+      ++out.candidates;
+      if (t == 0 && sp == 0) out.default_ms = ms;
+      if (ms < best) {
+        best = ms;
+        out.best = forced_;
+      }
+    }
+  forcing_ = false;
+  forced_ = JkShape{};
+  out.best_ms = best;
+  out.u_tile = out.best.u_tile > 0 ? out.best.u_tile : occupied_tile(nocc);
+  out.syrk_splits = out.best.syrk_splits > 0 ? out.best.syrk_splits : rule_splits;
+  TuningEntry e;
+  e.arch = arch_;
+  e.precision = static_cast<int>(p);
+  e.n = n_;
+  e.r = nocc;
+  e.ldq = ldq_;
+  e.shape = out.best;
+  e.ms = best;
+  record_shape(e);
+  return out;
 }
 
 std::vector<double> DfPlan::hcore() const {

@@ -11,6 +11,7 @@
 #include "common.hpp"
 #include "integrals.hpp"
 #include "jk.hpp"
+#include "tuning.hpp"
 
 namespace vqc {
 
@@ -45,6 +46,15 @@ struct ScfOutcome {
 
 struct JkResult {
   std::vector<double> J, K;
+};
+
+// Outcome of DfPlan::autotune: the fastest launch shape (0 fields: the rule's
+// choice), its build time and the rule's, both averages of HIP-event times.
+struct AutotuneResult {
+  JkShape best;
+  int u_tile = 0, syrk_splits = 0;  // the tile and split count the best shape ran with
+  double best_ms = 0.0, default_ms = 0.0;
+  int candidates = 0;
 };
 
 // (D + D^T) / 2 of a finite n x n density that is symmetric to 1e-10 of its
@@ -110,6 +120,14 @@ class DfPlan {
   void evaluate_occupied_into(const double* C, int nocc, double* J, double* K, void* scratch, std::size_t bytes);
   // Timing harness (see vibeqc_fock_plan_benchmark_occupied).
   std::array<double, 5> benchmark_occupied(const std::vector<double>& occupied, int nocc, unsigned repeat);
+  // Times the occupied J/K build of C (row-major n x nocc) at the plan's
+  // precision for the rule's launch shape and for alternative U tile widths
+  // (one to three i blocks) and SYRK split counts (half and twice the
+  // rule's), `repeat` builds each after one untimed build, and records the
+  // fastest in the tuning registry for this device, precision and shape.
+  AutotuneResult autotune(const std::vector<double>& occupied, int nocc, unsigned repeat);
+  // The launch shape used for builds of r occupied columns at precision p.
+  JkShape shape_for(Precision p, int r) const;
 
   // Unrestricted J/K for spin densities Da, Db (unit occupation, host in/out).
   // With `gradient` non-null the builds run in FP64 and *gradient receives
@@ -173,6 +191,9 @@ class DfPlan {
   std::vector<double> nuclear_repulsion_gradient() const;
 
   int device_;
+  std::string arch_;  // device architecture name (gfx...), the tuning registry's key
+  bool forcing_ = false;  // autotune in progress: forced_ overrides the registry
+  JkShape forced_;
   hipStream_t stream_ = nullptr;
   rocblas_handle blas_ = nullptr;
   bool owns_stream_ = true;

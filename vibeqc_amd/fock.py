@@ -380,6 +380,35 @@ class FockPlan:
             )
         return {"exchange_u_ms": ms[0], "exchange_syrk_ms": ms[1], "coulomb_ms": ms[2], "build_ms": ms[3], "wall_ms": ms[4]}
 
+    def autotune(self, occupied, repeat: int = 3) -> dict:
+        """Time the occupied J/K build of ``occupied`` (AO x nocc) at the
+        plan's precision for the default launch shape and its alternatives
+        (U i-tile width, SYRK split-K count), and record the fastest in the
+        process-wide tuning registry (``vibeqc_amd.tuning``), which later
+        builds of this shape use.  Returns the chosen ``u_tile`` and
+        ``syrk_splits`` with the best and default build times (ms)."""
+        n = self.nao
+        c = np.array(occupied, dtype=np.float64, order="C", copy=True)
+        if c.ndim != 2 or c.shape[0] != n or not 0 < c.shape[1] <= n:
+            raise ValueError("occupied factor must be (AO, nocc) with 0 < nocc <= AO")
+        shape = (ct.c_int32 * 2)()
+        ms = (ct.c_double * 2)()
+        with self._lock:
+            self._ensure_open()
+            self._check(
+                self._lib.vibeqc_fock_plan_autotune(self._handle, _native.pointer(c), n, c.shape[1], int(repeat), shape, ms)
+            )
+        return {"u_tile": shape[0], "syrk_splits": shape[1], "best_ms": ms[0], "default_ms": ms[1]}
+
+    def launch_shape(self, nocc: int) -> dict:
+        """The U i-tile width and SYRK split count that builds with ``nocc``
+        occupied columns use at the current precision."""
+        shape = (ct.c_int32 * 2)()
+        with self._lock:
+            self._ensure_open()
+            self._check(self._lib.vibeqc_fock_plan_launch_shape(self._handle, int(nocc), shape))
+        return {"u_tile": shape[0], "syrk_splits": shape[1]}
+
     def solve(
         self,
         *,
