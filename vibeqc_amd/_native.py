@@ -341,7 +341,7 @@ EXPORTS = {
     ),
     "vibeqc_fock_plan_shape": (ct.c_int32, [ct.c_void_p, ct.POINTER(ct.c_uint64)]),
     "vibeqc_fock_plan_streaming": (ct.c_int32, [ct.c_void_p, ct.POINTER(ct.c_uint64)]),
-    "vibeqc_fock_plan_last_solve_stats": (ct.c_int32, [ct.c_void_p, ct.POINTER(ct.c_uint64)]),
+    "vibeqc_fock_plan_last_solve_stats": (ct.c_int32, [ct.c_void_p, ct.POINTER(ct.c_uint64), ct.c_uint64]),
     "vibeqc_fock_plan_autotune": (
         ct.c_int32,
         [ct.c_void_p, _DP, ct.c_uint64, ct.c_uint64, ct.c_uint32, ct.POINTER(ct.c_int32), _DP],

@@ -60,7 +60,8 @@ struct generativeqc_fock_plan {
   generativeqc_fock_spec resolved{};
   double screening = 1e-12, threshold = 1e-10;
   uint64_t budget = 0;
-  uint64_t solve_stats[3] = {0, 0, 0};  // iterations, reduced-precision iterations, purified densities
+  // iterations, reduced-precision iterations, purified densities, incremental FP64-stage builds
+  uint64_t solve_stats[4] = {0, 0, 0, 0};
   std::string detail;
 };
 
@@ -478,6 +479,7 @@ generativeqc_status generativeqc_fock_plan_solve(generativeqc_fock_plan* plan,
     plan->solve_stats[0] = r.iterations;
     plan->solve_stats[1] = r.reduced_iterations;
     plan->solve_stats[2] = r.purified_iterations;
+    plan->solve_stats[3] = r.incremental_builds;
     if (!r.converged) {
       plan->detail = "Fock SCF did not converge in " + std::to_string(r.iterations) + " iterations";
       return GENERATIVEQC_STATUS_NOT_CONVERGED;
@@ -657,9 +659,10 @@ generativeqc_status vibeqc_fock_plan_streaming(const generativeqc_fock_plan* pla
   return GENERATIVEQC_STATUS_SUCCESS;
 }
 
-generativeqc_status vibeqc_fock_plan_last_solve_stats(const generativeqc_fock_plan* plan, uint64_t* stats3) {
-  if (!plan || !stats3) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
-  std::copy(plan->solve_stats, plan->solve_stats + 3, stats3);
+generativeqc_status vibeqc_fock_plan_last_solve_stats(const generativeqc_fock_plan* plan, uint64_t* stats,
+                                                      uint64_t count) {
+  if (!plan || !stats || count == 0) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
+  std::copy(plan->solve_stats, plan->solve_stats + std::min<uint64_t>(count, 4), stats);
   return GENERATIVEQC_STATUS_SUCCESS;
 }
 

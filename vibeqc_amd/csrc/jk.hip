@@ -745,12 +745,17 @@ __device__ __forceinline__ void syrk_tile_of(int t, int ncb, int ratio, int& rb,
 // 128 x 64 = 8 x 4 MFMA 16x16x32 tiles.  Per k-step of 32 a wave reads 12
 // fragment pairs from LDS for 96 MFMAs.  The next k-step is staged in
 // registers while the current one runs (one LDS buffer, two barriers).
+// With Vh/Vl non-null (cross mode) the column panels come from V instead of W
+// and the tiles cover the whole n x n grid (ntiles = ncb^2): partial slices
+// of X Y^T, every element written.
 __global__ __launch_bounds__(512) void exchange_syrk_bf16x3_kernel(const __bf16* __restrict__ Wh,
                                                                   const __bf16* __restrict__ Wl, std::size_t kd,
                                                                   int n, int ntiles, int ncb, int nwork,
                                                                   std::size_t kbegin, std::size_t kend,
                                                                   std::size_t kchunk, double sign,
-                                                                  float* __restrict__ partial) {
+                                                                  float* __restrict__ partial,
+                                                                  const __bf16* __restrict__ Vh,
+                                                                  const __bf16* __restrict__ Vl) {
   constexpr int T = kSyrkBfT, BK = 32, NT = 512, XF = 8, YF = 4;
   // Rows of 64 bytes with swizzled chunks (lds_chunk).
   __shared__ __bf16 Ah[T][BK], Al[T][BK];
@@ -759,11 +764,19 @@ __global__ __launch_bounds__(512) void exchange_syrk_bf16x3_kernel(const __bf16*
   const int work = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (work >= nwork) return;
   const int split = work / ntiles, tile = work % ntiles;
+  const bool cross = Vh != nullptr;
   int rb, cb;
-  syrk_tile_of(tile, ncb, 1, rb, cb);
+  if (cross) {
+    rb = tile / ncb;
+    cb = tile % ncb;
+  } else {
+    syrk_tile_of(tile, ncb, 1, rb, cb);
+  }
   const int m0 = rb * T, l0 = cb * T;
   // A diagonal tile multiplies a panel by itself: one gather, one LDS copy.
-  const bool diag = m0 == l0;
+  const bool diag = !cross && m0 == l0;
+  const __bf16* __restrict__ Ch = cross ? Vh : Wh;  // column-panel operand
+  const __bf16* __restrict__ Cl = cross ? Vl : Wl;
   const std::size_t k0 = kbegin + static_cast<std::size_t>(split) * kchunk;
   const std::size_t k1 = k0 + kchunk < kend ? k0 + kchunk : kend;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -789,8 +802,8 @@ __global__ __launch_bounds__(512) void exchange_syrk_bf16x3_kernel(const __bf16*
       }
       if (!diag && gb < n) {
         const std::size_t off = static_cast<std::size_t>(gb) * kd + kb + kc;
-        rbv[j][0] = *reinterpret_cast<const bf8*>(Wh + off);
-        rbv[j][1] = *reinterpret_cast<const bf8*>(Wl + off);
+        rbv[j][0] = *reinterpret_cast<const bf8*>(Ch + off);
+        rbv[j][1] = *reinterpret_cast<const bf8*>(Cl + off);
       }
     }
   };
@@ -849,7 +862,7 @@ __global__ __launch_bounds__(512) void exchange_syrk_bf16x3_kernel(const __bf16*
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int mu = m0 + wr * 16 * XF + i * 16 + 4 * (lane >> 4) + v;
-        if (mu < n && lam < n && lam <= mu) out[static_cast<std::size_t>(mu) * n + lam] = fsign * acc[i][j][v];
+        if (mu < n && lam < n && (cross || lam <= mu)) out[static_cast<std::size_t>(mu) * n + lam] = fsign * acc[i][j][v];
       }
     }
 }
@@ -962,6 +975,25 @@ __global__ void reduce_exchange_kernel(const P* __restrict__ partial, int slices
   for (int k = 0; k < slices; ++k) s += static_cast<double>(partial[static_cast<std::size_t>(k) * n * n + e]);
   K[static_cast<std::size_t>(mu) * n + lam] = s;
   K[static_cast<std::size_t>(lam) * n + mu] = s;
+}
+
+// K = Kref + W + W^T with W the sum of the cross partial slices (FP64 sum in
+// slice order); K is symmetric by construction.
+__global__ void reduce_cross_kernel(const float* __restrict__ partial, int slices, int n,
+                                    const double* __restrict__ Kref, double* __restrict__ K) {
+  const std::size_t e = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<std::size_t>(n) * n) return;
+  const int mu = static_cast<int>(e / n), lam = static_cast<int>(e % n);
+  if (lam > mu) return;
+  const std::size_t et = static_cast<std::size_t>(lam) * n + mu;
+  double s = 0.0, t = 0.0;
+  for (int k = 0; k < slices; ++k) {
+    s += static_cast<double>(partial[static_cast<std::size_t>(k) * n * n + e]);
+    t += static_cast<double>(partial[static_cast<std::size_t>(k) * n * n + et]);
+  }
+  const double v = Kref[e] + (s + t);
+  K[e] = v;
+  K[et] = v;
 }
 
 __global__ void convert_kernel(const double* __restrict__ in, float* __restrict__ out, std::size_t count) {
@@ -1221,7 +1253,8 @@ void exchange_syrk(Precision prec, const void* U, std::size_t ustride, int n, in
       const __bf16* wh = static_cast<const __bf16*>(U);
       const __bf16* wl = wh + static_cast<std::size_t>(n) * ustride;
       hipLaunchKernelGGL(exchange_syrk_bf16x3_kernel, dim3(grid), dim3(512), 0, s, wh, wl, ustride, n, tiles, ncb,
-                         nwork, kb, ke, chunk, sign, out32);
+                         nwork, kb, ke, chunk, sign, out32, static_cast<const __bf16*>(nullptr),
+                         static_cast<const __bf16*>(nullptr));
     } else if (prec == Precision::FP64) {
       hipLaunchKernelGGL(exchange_syrk_kernel<double>, dim3(grid), dim3(512), 0, s, static_cast<const double*>(U),
                          ustride, n, tiles, ncb, nwork, kb, ke, chunk, sign, out64);
@@ -1245,6 +1278,32 @@ void exchange_syrk(Precision prec, const void* U, std::size_t ustride, int n, in
     hipLaunchKernelGGL(reduce_exchange_kernel<float>, dim3((e + 255) / 256), dim3(256), 0, s,
                        reinterpret_cast<const float*>(partial), slices, n, K);
   VQC_LAUNCH("reduce_exchange_kernel");
+}
+
+void exchange_cross_bf16x3(const void* X, const void* Y, std::size_t ustride, int n, int r, int ldq,
+                           const double* Kref, double* K, double* partial, hipStream_t s) {
+  const std::size_t kd = static_cast<std::size_t>(r) * ldq;
+  if (ustride < kd) throw std::invalid_argument("U row stride shorter than r x ldq");
+  const int cap = syrk_cap(n, 0);
+  const int ncb = (n + kSyrkBfT - 1) / kSyrkBfT;
+  const int tiles = ncb * ncb;
+  int splits = syrk_splits(tiles, cap, syrk_slots(Precision::BF16X3));
+  const std::size_t chunk = round_up((kd + splits - 1) / splits, 32);
+  splits = static_cast<int>((kd + chunk - 1) / chunk);
+  if (splits > 2 * cap) throw std::length_error("cross partial slices exceed the partial buffer");
+  const int nwork = tiles * splits;
+  const unsigned grid = static_cast<unsigned>(round_up(nwork, 8));
+  const __bf16* xh = static_cast<const __bf16*>(X);
+  const __bf16* xl = xh + static_cast<std::size_t>(n) * ustride;
+  const __bf16* yh = static_cast<const __bf16*>(Y);
+  const __bf16* yl = yh + static_cast<std::size_t>(n) * ustride;
+  float* out32 = reinterpret_cast<float*>(partial);
+  hipLaunchKernelGGL(exchange_syrk_bf16x3_kernel, dim3(grid), dim3(512), 0, s, xh, xl, ustride, n, tiles, ncb, nwork,
+                     std::size_t{0}, kd, chunk, 1.0, out32, yh, yl);
+  VQC_LAUNCH("exchange_syrk_bf16x3_kernel (cross)");
+  const std::size_t e = static_cast<std::size_t>(n) * n;
+  hipLaunchKernelGGL(reduce_cross_kernel, dim3((e + 255) / 256), dim3(256), 0, s, out32, splits, n, Kref, K);
+  VQC_LAUNCH("reduce_cross_kernel");
 }
 
 namespace {

@@ -72,6 +72,14 @@ void exchange_syrk(Precision prec, const void* U, std::size_t ustride, int n, in
 // The split-K slices per run that the occupancy rule picks for this precision
 // and n (whole waves of resident blocks on 256 CUs), under the same cap.
 int exchange_default_splits(Precision prec, int n, std::size_t partial_capacity = 0);
+// K = Kref + W + W^T (n x n, FP64, both triangles written) with W[mu][lam] =
+// sum_k X[mu][k] Y[lam][k] over k < r ldq, for two BF16X3 U buffers X and Y
+// (hi/lo planes, rows ustride apart, as occupied_u writes them).  With X and Y
+// the U of (C + Cref)/2 and C - Cref, W + W^T is K(C C^T - Cref Cref^T):
+// an exchange update whose BF16X3 rounding scales with |C - Cref|.  `partial`
+// as for exchange_syrk (FP32 slices, n^2 floats each).
+void exchange_cross_bf16x3(const void* X, const void* Y, std::size_t ustride, int n, int r, int ldq,
+                           const double* Kref, double* K, double* partial, hipStream_t s);
 // The most split-K slices per run the partial buffer holds (at most 128).
 int exchange_max_splits(int n, std::size_t partial_capacity = 0);
 // C (row-major n x nocc) -> factor n x ldc scaled by sqrt(2), zero padded, and

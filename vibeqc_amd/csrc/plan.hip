@@ -157,7 +157,8 @@ std::size_t DfPlan::device_bytes() const {
   return boys_.bytes() + S_.bytes() + H_.bytes() + B64_.bytes() + B32_.bytes() + dpacked_.bytes() +
          rho_.bytes() + rho_partial_.bytes() + J_.bytes() + K_.bytes() + syrk_partial_.bytes() + D_.bytes() +
          c64_.bytes() + c32_.bytes() + ct_hi_.bytes() + ct_lo_.bytes() + U_.bytes() + L_.bytes() + G_.bytes() +
-         Ablk_.bytes() + Bblk_.bytes() + Jtmp_.bytes() + Ktmp_.bytes() + Ublk_.bytes();
+         Ablk_.bytes() + Bblk_.bytes() + Jtmp_.bytes() + Ktmp_.bytes() + Ublk_.bytes() + Kref_.bytes() + Cref_.bytes() +
+         facA_.bytes() + facB_.bytes();
 }
 
 std::size_t DfPlan::resident_bytes() const {
@@ -410,7 +411,9 @@ void DfPlan::ensure_factor(int r) {
   ct_hi_.resize(static_cast<std::size_t>(ldc_) * ldct);
   ct_lo_.resize(static_cast<std::size_t>(ldc_) * ldct);
   U_.release();
-  if (!streamed_) U_.resize(static_cast<std::size_t>(n_) * r * ldq_ * sizeof(double));  // streamed: Ublk_ per block
+  // FP64 U (r ldq doubles per mu), or the two BF16X3 U of an incremental
+  // build (2 x 4 (r ldq + 64) bytes per mu).  Streamed plans: Ublk_ per block.
+  if (!streamed_) U_.resize(static_cast<std::size_t>(n_) * (static_cast<std::size_t>(r) * ldq_ + 64) * sizeof(double));
   ops_ = OccupiedOperands{};
   ops_.n = n_;
   ops_.ldc = ldc_;
@@ -451,6 +454,56 @@ void DfPlan::build_jk(Precision p, const double* D, int r, int r_pos) {
     }
     coulomb_matrix(p, B, rho_.data(), npair_, ldq_, n_, J_.data(), stream_);
   }
+}
+
+bool DfPlan::incremental_enabled(int r) const {
+  if (streamed_ || !opts_.exchange || prec_ != Precision::BF16X3) return false;
+  if (const char* v = std::getenv("VIBEQC_INCREMENTAL_K")) {
+    if (v[0] == '0') return false;
+    if (v[0] == '1') return true;
+  }
+  return static_cast<double>(n_) * r * ldq_ >= 2e8;
+}
+
+namespace {
+// A = (C + R) / 2, B = C - R over `count` elements.
+__global__ void half_sum_difference_kernel(const double* __restrict__ C, const double* __restrict__ R,
+                                           std::size_t count, double* __restrict__ A, double* __restrict__ B) {
+  const std::size_t stride = static_cast<std::size_t>(gridDim.x) * blockDim.x;
+  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
+    const double c = C[i], r = R[i];
+    A[i] = 0.5 * (c + r);
+    B[i] = c - r;
+  }
+}
+}  // namespace
+
+void DfPlan::build_jk_incremental(int r) {
+  ensure_reduced();
+  const std::size_t nn = static_cast<std::size_t>(n_) * n_;
+  const std::size_t fc = static_cast<std::size_t>(n_) * ldc_;
+  if (Kref_.size() != nn || Cref_.size() != fc) throw std::logic_error("incremental build without an anchor");
+  facA_.resize(fc);
+  facB_.resize(fc);
+  // J of the current density in FP64 (one pass over B for rho, one for J).
+  coulomb_only(Precision::FP64, D_.data());
+  hipLaunchKernelGGL(half_sum_difference_kernel, dim3(static_cast<unsigned>(std::min<std::size_t>((fc + 255) / 256, 4096))),
+                     dim3(256), 0, stream_, c64_.data(), Cref_.data(), fc, facA_.data(), facB_.data());
+  VQC_LAUNCH("half_sum_difference_kernel");
+  const std::size_t ustride = u_row_stride(Precision::BF16X3, r, ldq_);
+  char* ua = U_.data();
+  char* ub = ua + static_cast<std::size_t>(n_) * ustride * 2 * sizeof(uint16_t);
+  if (static_cast<std::size_t>(ub - ua) * 2 > U_.size()) throw std::logic_error("U buffer too small for two BF16X3 U");
+  OccupiedOperands op = ops_;
+  op.r = r;
+  op.ustride = ustride;
+  op.c64 = facA_.data();
+  prepare_occupied(Precision::BF16X3, op, stream_);
+  occupied_u(Precision::BF16X3, B32_.data(), op, ldq_, ua, nullptr, nullptr, stream_);
+  op.c64 = facB_.data();
+  prepare_occupied(Precision::BF16X3, op, stream_);
+  occupied_u(Precision::BF16X3, B32_.data(), op, ldq_, ub, nullptr, nullptr, stream_);
+  exchange_cross_bf16x3(ua, ub, ustride, n_, r, ldq_, Kref_.data(), K_.data(), syrk_partial_.data(), stream_);
 }
 
 // Streamed plans (the reference's budgeted DF planner,
@@ -1094,7 +1147,11 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   const bool sp2_usable = nocc > 0 && nocc < n && pivoted_cholesky_fits(n, nocc);
   DeviceArray<double> G(sp2_usable ? static_cast<std::size_t>(n) * nocc : 1), sp2_state(kSp2StateSize),
       sp2_partial(sp2_partial_size(n));
-  DeviceArray<int> sp2_status(1), proj_status(1);
+  // [0] the factor's status (0: usable), [1] the projection's (0: accepted).
+  DeviceArray<int> fac_status(2);
+  int* sp2_status = fac_status.data();
+  int* proj_status = fac_status.data() + 1;
+  bool projected = false;  // the last purified factor came from the projection
   // The previous iteration's projector factor G: once one was accepted, the
   // next is tried as a projection of it (a few GEMMs) before the
   // single-workgroup pivoted Cholesky.
@@ -1134,11 +1191,13 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
     gemm(rocblas_operation_none, rocblas_operation_none, n, Fm, n, X.data(), n, T1.data());
     gemm(rocblas_operation_none, rocblas_operation_none, n, X.data(), n, T1.data(), n, T2.data());
     bool purified = false;
+    projected = false;
     if (!canonical && sp2_usable && purify()) {
       if (have_factor)
-        projected_factor(blas_, V.data(), n, nocc, G.data(), projA.data(), projM.data(), proj_info.data(),
-                         proj_status.data(), stream_);
-      pivoted_cholesky(V.data(), n, nocc, G.data(), sp2_status.data(), stream_, have_factor ? proj_status.data() : nullptr);
+        projected_factor(blas_, V.data(), n, nocc, G.data(), projA.data(), projM.data(), proj_info.data(), proj_status,
+                         stream_);
+      projected = have_factor;  // confirmed by the status in purified_ok
+      pivoted_cholesky(V.data(), n, nocc, G.data(), sp2_status, stream_, have_factor ? proj_status : nullptr);
       VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, n, nocc, n, &one, X.data(), n,
                              G.data(), n, &zero, W.data(), n));
       purified = true;
@@ -1153,10 +1212,11 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   };
   // Only an accepted factor seeds the next iteration's projection.
   auto purified_ok = [&]() {
-    int status = 1;
-    sp2_status.download(&status, 1, stream_);
-    have_factor = status == 0;
-    return status == 0;
+    int status[2] = {1, 1};
+    fac_status.download(status, 2, stream_);
+    have_factor = status[0] == 0;
+    projected = projected && status[0] == 0 && status[1] == 0;
+    return status[0] == 0;
   };
   ScfOutcome out;
   // Initial density: core Hamiltonian guess (src/scf/initial_guess/density.cpp:635-672) or caller seed.
@@ -1191,6 +1251,23 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   double best_rms = std::numeric_limits<double>::infinity();
   int stalled = 0;
   bool in_reduced = reduced;
+  // Incremental FP64 stage (large BF16X3 solves): each full FP64 build is an
+  // anchor (its K and factor kept); the next builds take K from the anchor
+  // plus a BF16X3 update whose rounding scales with the factor's change since
+  // then, as long as every new factor since the anchor is the projection of
+  // the one before (so the columns stay aligned and that change stays small).
+  // The update's rounding is relative to the change since the anchor, so the
+  // anchor is renewed once the density change has fallen kReanchor below its
+  // value at the anchor (measured on (H2O)64: without this the energy wanders
+  // at 1e-9 Eh, above the 1e-10 criterion, until the run of kIncrementalRun
+  // builds ends).  J is always FP64, and the finalisation below rebuilds in
+  // full FP64.
+  constexpr int kIncrementalRun = 8;
+  constexpr double kReanchor = 1e-2;
+  const bool incremental = reduced && incremental_enabled(nocc);
+  bool anchored = false, aligned = false;
+  int since_anchor = 0;
+  double anchor_rms = 0.0;
   double previous = std::numeric_limits<double>::infinity();
   bool converged = false;
   unsigned it = 0;
@@ -1204,7 +1281,26 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
       rp = packed & 0xffff;
       rank_now = -1;
     }
-    build_jk(p, D_.data(), r, rp);
+    const bool full_rank = r == nocc && rp == nocc;
+    if (!in_reduced && incremental && anchored && aligned && full_rank && since_anchor < kIncrementalRun &&
+        rms >= kReanchor * anchor_rms) {
+      build_jk_incremental(nocc);
+      ++since_anchor;
+      ++out.incremental_builds;
+    } else {
+      build_jk(p, D_.data(), r, rp);
+      anchored = false;
+      if (!in_reduced && incremental && full_rank) {
+        Kref_.resize(nn);
+        Cref_.resize(static_cast<std::size_t>(n) * ldc_);
+        copy_device(Kref_.data(), K_.data(), nn, stream_);
+        copy_device(Cref_.data(), c64_.data(), Cref_.size(), stream_);
+        anchored = true;
+        aligned = true;
+        since_anchor = 0;
+        anchor_rms = rms;  // the change that produced this density
+      }
+    }
     ++out.fock_builds;
     if (in_reduced) ++out.reduced_iterations;
     assemble_fock(H_.data(), J_.data(), K_.data(), opts_.cj, opts_.ck, nn, F.data(), stream_);
@@ -1249,6 +1345,8 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
         scalars.download(hs, 4, stream_);
       }
     }
+    // The new factor continues the aligned chain only as a projection of the last.
+    aligned = aligned && purified && projected;
     sync();
     if (eigh_failed(hs)) throw NumericalFailure("Fock eigensolver did not converge");
     energy = 0.5 * (hs[0] + hs[1]) + enuc_;

@@ -39,6 +39,7 @@ struct ScfOutcome {
   unsigned iterations = 0;
   unsigned reduced_iterations = 0;  // iterations run at reduced precision
   unsigned purified_iterations = 0;  // new densities from SP2 purification (the rest diagonalise)
+  unsigned incremental_builds = 0;   // FP64-stage builds with K from an anchor plus a BF16X3 update
   uint64_t fock_builds = 0;
   std::vector<double> density;
   std::vector<double> forces;  // 3 natom, -dE/dR, when requested
@@ -161,6 +162,15 @@ class DfPlan {
   // J/K on the device: D (n x n) and the factor already in c64_ with r
   // columns, r_pos of them positive.  Results in J_, K_.
   void build_jk(Precision p, const double* D, int r, int r_pos);
+  // FP64-stage build of a mixed BF16X3 solve from the anchor (Kref_, Cref_):
+  // J in FP64 from D_, K = Kref + K(C C^T - Cref Cref^T) with the update in
+  // BF16X3 from the U of (C + Cref)/2 and C - Cref (C = the factor in c64_,
+  // r columns, all positive).  Results in J_, K_; overwrites U_.
+  void build_jk_incremental(int r);
+  // Whether the FP64 stage of a mixed solve with r occupied columns uses
+  // incremental builds (VIBEQC_INCREMENTAL_K = 0 never, 1 always, else for
+  // n r ldq >= 2e8, where the FP64 build dominates the solve).
+  bool incremental_enabled(int r) const;
   void upload_factor(const std::vector<double>& occupied, int nocc);
   int factorise_density(const double* D);
   void sync();
@@ -239,6 +249,9 @@ class DfPlan {
   bool streamed_ = false;
   bool g_lower_ = false;
   DeviceArray<double> G_, Ablk_, Bblk_, Jtmp_, Ktmp_;
+  // Anchor of incremental FP64-stage builds: K and the factor of the last
+  // full FP64 build, and the two update factors.
+  DeviceArray<double> Kref_, Cref_, facA_, facB_;
   DeviceArray<char> Ublk_;
   std::vector<int> tile_shells_;  // shell boundaries of the row tiles
   int qblk_ = 0;                  // auxiliary columns per block (multiple of 128)

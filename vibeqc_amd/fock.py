@@ -255,11 +255,13 @@ class FockPlan:
 
     def last_solve_stats(self) -> dict:
         """Statistics of the last solve(): iterations, iterations with
-        reduced-precision Fock builds, new densities from purification."""
-        out = (ct.c_uint64 * 3)()
+        reduced-precision Fock builds, new densities from purification, and
+        FP64-stage builds of a mixed solve done incrementally."""
+        out = (ct.c_uint64 * 4)()
         with self._lock:
-            self._check(self._lib.vibeqc_fock_plan_last_solve_stats(self._handle, out))
-        return {"iterations": out[0], "reduced_iterations": out[1], "purified_densities": out[2]}
+            self._check(self._lib.vibeqc_fock_plan_last_solve_stats(self._handle, out, 4))
+        return {"iterations": out[0], "reduced_iterations": out[1], "purified_densities": out[2],
+                "incremental_builds": out[3]}
 
     def one_electron(self):
         """(S, Hcore) as nbf x nbf arrays."""
