@@ -745,9 +745,10 @@ __device__ __forceinline__ void syrk_tile_of(int t, int ncb, int ratio, int& rb,
 // 128 x 64 = 8 x 4 MFMA 16x16x32 tiles.  Per k-step of 32 a wave reads 12
 // fragment pairs from LDS for 96 MFMAs.  The next k-step is staged in
 // registers while the current one runs (one LDS buffer, two barriers).
-// With Vh/Vl non-null (cross mode) the column panels come from V instead of W
-// and the tiles cover the whole n x n grid (ntiles = ncb^2): partial slices
-// of X Y^T, every element written.
+// CROSS: the column panels come from V instead of W and the tiles cover the
+// whole n x n grid (ntiles = ncb^2): partial slices of W V^T, every element
+// written.
+template <bool CROSS>
 __global__ __launch_bounds__(512) void exchange_syrk_bf16x3_kernel(const __bf16* __restrict__ Wh,
                                                                   const __bf16* __restrict__ Wl, std::size_t kd,
                                                                   int n, int ntiles, int ncb, int nwork,
@@ -764,9 +765,9 @@ __global__ __launch_bounds__(512) void exchange_syrk_bf16x3_kernel(const __bf16*
   const int work = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (work >= nwork) return;
   const int split = work / ntiles, tile = work % ntiles;
-  const bool cross = Vh != nullptr;
+  constexpr bool cross = CROSS;
   int rb, cb;
-  if (cross) {
+  if constexpr (CROSS) {
     rb = tile / ncb;
     cb = tile % ncb;
   } else {
@@ -1252,9 +1253,8 @@ void exchange_syrk(Precision prec, const void* U, std::size_t ustride, int n, in
     if (prec == Precision::BF16X3) {
       const __bf16* wh = static_cast<const __bf16*>(U);
       const __bf16* wl = wh + static_cast<std::size_t>(n) * ustride;
-      hipLaunchKernelGGL(exchange_syrk_bf16x3_kernel, dim3(grid), dim3(512), 0, s, wh, wl, ustride, n, tiles, ncb,
-                         nwork, kb, ke, chunk, sign, out32, static_cast<const __bf16*>(nullptr),
-                         static_cast<const __bf16*>(nullptr));
+      hipLaunchKernelGGL(exchange_syrk_bf16x3_kernel<false>, dim3(grid), dim3(512), 0, s, wh, wl, ustride, n, tiles,
+                         ncb, nwork, kb, ke, chunk, sign, out32, wh, wl);
     } else if (prec == Precision::FP64) {
       hipLaunchKernelGGL(exchange_syrk_kernel<double>, dim3(grid), dim3(512), 0, s, static_cast<const double*>(U),
                          ustride, n, tiles, ncb, nwork, kb, ke, chunk, sign, out64);
@@ -1298,7 +1298,7 @@ void exchange_cross_bf16x3(const void* X, const void* Y, std::size_t ustride, in
   const __bf16* yh = static_cast<const __bf16*>(Y);
   const __bf16* yl = yh + static_cast<std::size_t>(n) * ustride;
   float* out32 = reinterpret_cast<float*>(partial);
-  hipLaunchKernelGGL(exchange_syrk_bf16x3_kernel, dim3(grid), dim3(512), 0, s, xh, xl, ustride, n, tiles, ncb, nwork,
+  hipLaunchKernelGGL(exchange_syrk_bf16x3_kernel<true>, dim3(grid), dim3(512), 0, s, xh, xl, ustride, n, tiles, ncb, nwork,
                      std::size_t{0}, kd, chunk, 1.0, out32, yh, yl);
   VQC_LAUNCH("exchange_syrk_bf16x3_kernel (cross)");
   const std::size_t e = static_cast<std::size_t>(n) * n;
