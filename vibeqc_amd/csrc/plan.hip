@@ -1393,7 +1393,17 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
     return out;
   }
   // Finalisation (mean_field_driver.cpp:181-199): rebuild, diagonalise, rebuild.
-  build_jk(Precision::FP64, D_.data(), nocc, nocc);
+  // The first rebuild only orients the final orbitals, so an incremental
+  // build from an anchor the converged factor is still aligned with serves
+  // (its rounding, relative to the factor's change since the anchor, is far
+  // below the converged density change); the energy, density and forces come
+  // from the full FP64 rebuild.
+  if (incremental && anchored && aligned && rank_now == nocc) {
+    build_jk_incremental(nocc);
+    ++out.incremental_builds;
+  } else {
+    build_jk(Precision::FP64, D_.data(), nocc, nocc);
+  }
   assemble_fock(H_.data(), J_.data(), K_.data(), opts_.cj, opts_.ck, nn, F.data(), stream_);
   new_density(F.data(), D_.data(), true);
   build_jk(Precision::FP64, D_.data(), nocc, nocc);
