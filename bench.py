@@ -440,7 +440,8 @@ def _scf_stats(plan, result, seconds):
     st = plan.last_solve_stats()
     return {"energy": result.energy, "iterations": result.iterations, "fock_builds": result.fock_builds,
             "reduced_iterations": st["reduced_iterations"], "purified_densities": st["purified_densities"],
-            "incremental_builds": st["incremental_builds"], "scf_s": seconds}
+            "incremental_builds": st["incremental_builds"], "single_term_builds": st["single_term_builds"],
+            "scf_s": seconds}
 
 
 def _scf_timing(precision):

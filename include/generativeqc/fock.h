@@ -229,11 +229,13 @@ GENERATIVEQC_API generativeqc_status vibeqc_fock_plan_streaming(const generative
                                                                 uint64_t* streaming3);
 
 /* Extension (no counterpart in the reference): statistics of the plan's last
- * generativeqc_fock_plan_solve, the first min(count, 4) of: [0] SCF
+ * generativeqc_fock_plan_solve, the first min(count, 5) of: [0] SCF
  * iterations, [1] of them with reduced-precision Fock builds, [2] new
  * densities obtained by purification (the rest by diagonalisation), [3]
  * FP64-stage builds of a BF16X3 solve done incrementally (exchange from the
- * last full FP64 build plus a BF16X3 update of the occupied factor's change). */
+ * last full FP64 build plus a BF16X3 update of the occupied factor's change),
+ * [4] BF16X3-stage builds with one bf16 product per logical one (the first
+ * iterations of a large BF16X3 solve). */
 GENERATIVEQC_API generativeqc_status vibeqc_fock_plan_last_solve_stats(const generativeqc_fock_plan* plan,
                                                                        uint64_t* stats, uint64_t count);
 

@@ -1,12 +1,12 @@
-"""Incremental FP64-stage builds of a mixed (BF16X3) SCF on the device.
+"""The large-system schedule of a mixed (BF16X3) SCF on the device.
 
-The FP64 refinement of a mixed solve may take K from the last full FP64 build
-plus a BF16X3 update of the occupied factor's change (used by default for
-large systems; VIBEQC_INCREMENTAL_K=1 forces it, =0 disables it).  Forced on
-the (H2O)16 GPU4PySCF fixture: the energy stays within 1e-8 Eh and the forces
-within 1e-7 Eh/bohr of the fixture, within 1e-10 Eh / 1e-8 Eh/bohr of the same
-solve without incremental builds, and some FP64-stage builds are incremental.
-FP64 solves never use them."""
+For large systems (VIBEQC_INCREMENTAL_K=1 forces it, =0 disables it) a mixed
+solve starts with single-term BF16 builds while the density change is above
+1e-2, and its FP64 refinement takes K from the last full FP64 build plus a
+BF16X3 update of the occupied factor's change.  Forced on the (H2O)16
+GPU4PySCF fixture: the energy stays within 1e-8 Eh and the forces within 1e-7
+Eh/bohr of the fixture, within 1e-10 Eh / 1e-7 Eh/bohr of the same solve
+without it, and both kinds of builds occur.  FP64 solves never use them."""
 
 from __future__ import annotations
 
@@ -46,14 +46,18 @@ def test_incremental_refinement_matches_fixture_and_full_builds(monkeypatch):
     print(f"(H2O)16 mixed: full-build refinement E {r0.energy:.12f} ({s0}), incremental E {r1.energy:.12f} ({s1}); "
           f"E1 - Eref {r1.energy - meta['energy_hartree']:.2e}, max |F1 - Fref| {np.abs(f1 - ref_f).max():.2e}, "
           f"max |F1 - F0| {np.abs(f1 - f0).max():.2e}")
-    assert s0["incremental_builds"] == 0
-    assert s1["incremental_builds"] >= 1
+    assert s0["incremental_builds"] == 0 and s0["single_term_builds"] == 0
+    assert s1["incremental_builds"] >= 1 and s1["single_term_builds"] >= 1
+    assert s1["single_term_builds"] < s1["reduced_iterations"]  # three-term builds follow them
     # Incremental builds replace FP64-stage builds after an anchor (a full
     # FP64 build), plus at most the first finalisation rebuild.
     assert s1["incremental_builds"] <= s1["iterations"] - s1["reduced_iterations"]
     assert abs(r1.energy - meta["energy_hartree"]) < 1e-8
     assert np.abs(f1 - ref_f).max() < 1e-7
+    # Different schedules converge to densities that agree to the density
+    # tolerance (1e-8 RMS), which moves forces (first order in the density)
+    # by a few 1e-8 Eh/bohr; energies (second order) agree far closer.
     assert abs(r1.energy - r0.energy) < 1e-10
-    assert np.abs(f1 - f0).max() < 1e-8
-    assert stats64["incremental_builds"] == 0 and stats64["reduced_iterations"] == 0
+    assert np.abs(f1 - f0).max() < 1e-7
+    assert stats64["incremental_builds"] == 0 and stats64["reduced_iterations"] == 0 and stats64["single_term_builds"] == 0
     assert abs(r64.energy - meta["energy_hartree"]) < 1e-8

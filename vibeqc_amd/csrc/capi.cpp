@@ -60,8 +60,9 @@ struct generativeqc_fock_plan {
   generativeqc_fock_spec resolved{};
   double screening = 1e-12, threshold = 1e-10;
   uint64_t budget = 0;
-  // iterations, reduced-precision iterations, purified densities, incremental FP64-stage builds
-  uint64_t solve_stats[4] = {0, 0, 0, 0};
+  // iterations, reduced-precision iterations, purified densities, incremental FP64-stage builds,
+  // single-term BF16 builds
+  uint64_t solve_stats[5] = {0, 0, 0, 0, 0};
   std::string detail;
 };
 
@@ -480,6 +481,7 @@ generativeqc_status generativeqc_fock_plan_solve(generativeqc_fock_plan* plan,
     plan->solve_stats[1] = r.reduced_iterations;
     plan->solve_stats[2] = r.purified_iterations;
     plan->solve_stats[3] = r.incremental_builds;
+    plan->solve_stats[4] = r.single_term_builds;
     if (!r.converged) {
       plan->detail = "Fock SCF did not converge in " + std::to_string(r.iterations) + " iterations";
       return GENERATIVEQC_STATUS_NOT_CONVERGED;
@@ -662,7 +664,7 @@ generativeqc_status vibeqc_fock_plan_streaming(const generativeqc_fock_plan* pla
 generativeqc_status vibeqc_fock_plan_last_solve_stats(const generativeqc_fock_plan* plan, uint64_t* stats,
                                                       uint64_t count) {
   if (!plan || !stats || count == 0) return GENERATIVEQC_STATUS_INVALID_ARGUMENT;
-  std::copy(plan->solve_stats, plan->solve_stats + std::min<uint64_t>(count, 4), stats);
+  std::copy(plan->solve_stats, plan->solve_stats + std::min<uint64_t>(count, 5), stats);
   return GENERATIVEQC_STATUS_SUCCESS;
 }
 

@@ -545,7 +545,7 @@ __global__ __launch_bounds__(1024) void occupied_u_bf16x3_kernel(const float* __
 // per k-step.  8 waves x 32 Q rows per block (two blocks per CU).  The C^T
 // global loads of the next step are issued before its B gathers, so the wait
 // that precedes their LDS store never waits for the gathers.
-template <int NIW>
+template <int NIW, int TERMS>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NIW <= 3 ? 4 : 2))) void occupied_u_bf16x3_direct_kernel(
     const float* __restrict__ B, const __bf16* __restrict__ Cth, const __bf16* __restrict__ Ctl,
     __bf16* __restrict__ Uh, __bf16* __restrict__ Ul, std::size_t us, int n, int ldq, int ldct, int r,
@@ -662,10 +662,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NIW <= 3 ? 
         const bf8 bl = *reinterpret_cast<const bf8*>(&Yl[cur][row][kq]);
 #pragma unroll
         for (int x = 0; x < XF; ++x) acc[x][y] = mfma(ah[x], bh, acc[x][y]);
+        if constexpr (TERMS == 3) {
 #pragma unroll
-        for (int x = 0; x < XF; ++x) acc[x][y] = mfma(ah[x], bl, acc[x][y]);
+          for (int x = 0; x < XF; ++x) acc[x][y] = mfma(ah[x], bl, acc[x][y]);
 #pragma unroll
-        for (int x = 0; x < XF; ++x) acc[x][y] = mfma(al[x], bh, acc[x][y]);
+          for (int x = 0; x < XF; ++x) acc[x][y] = mfma(al[x], bh, acc[x][y]);
+        }
       }
     }
     // The other buffer was last read before the previous barrier.
@@ -748,7 +750,7 @@ __device__ __forceinline__ void syrk_tile_of(int t, int ncb, int ratio, int& rb,
 // CROSS: the column panels come from V instead of W and the tiles cover the
 // whole n x n grid (ntiles = ncb^2): partial slices of W V^T, every element
 // written.
-template <bool CROSS>
+template <bool CROSS, int TERMS>
 __global__ __launch_bounds__(512) void exchange_syrk_bf16x3_kernel(const __bf16* __restrict__ Wh,
                                                                   const __bf16* __restrict__ Wl, std::size_t kd,
                                                                   int n, int ntiles, int ncb, int nwork,
@@ -841,10 +843,12 @@ __global__ __launch_bounds__(512) void exchange_syrk_bf16x3_kernel(const __bf16*
       // Independent accumulators between dependent MFMAs keep the pipe full.
 #pragma unroll
       for (int j = 0; j < YF; ++j) acc[i][j] = mfma(ah, bh[j], acc[i][j]);
+      if constexpr (TERMS == 3) {
 #pragma unroll
-      for (int j = 0; j < YF; ++j) acc[i][j] = mfma(ah, bl[j], acc[i][j]);
+        for (int j = 0; j < YF; ++j) acc[i][j] = mfma(ah, bl[j], acc[i][j]);
 #pragma unroll
-      for (int j = 0; j < YF; ++j) acc[i][j] = mfma(al, bh[j], acc[i][j]);
+        for (int j = 0; j < YF; ++j) acc[i][j] = mfma(al, bh[j], acc[i][j]);
+      }
     }
     __syncthreads();
     if (more) {
@@ -1040,11 +1044,13 @@ void launch_u(Precision prec, const void* B, const OccupiedOperands& ops, int ld
     // Row-offset table (n + 32 entries) and the rho row as float.
     const std::size_t ddyn = static_cast<std::size_t>(ops.n + 32) * sizeof(std::size_t) +
                              (rho_part ? static_cast<std::size_t>(ops.n) * sizeof(float) : 0);
-    hipLaunchKernelGGL((occupied_u_bf16x3_direct_kernel<NIW>), grid, dim3(512), ddyn, s, static_cast<const float*>(B),
+    auto kernel = ops.terms == 1 ? occupied_u_bf16x3_direct_kernel<NIW, 1> : occupied_u_bf16x3_direct_kernel<NIW, 3>;
+    hipLaunchKernelGGL(kernel, grid, dim3(512), ddyn, s, static_cast<const float*>(B),
                        static_cast<const __bf16*>(ops.ct_hi), static_cast<const __bf16*>(ops.ct_lo),
                        static_cast<__bf16*>(U), static_cast<__bf16*>(U) + static_cast<std::size_t>(ops.n) * ops.ustride,
                        ops.ustride, ops.n, ldq, ops.ldct, ops.r, D, rho_part);
   } else {
+    if (ops.terms != 3) throw std::logic_error("single-term BF16 products need i tiles of at most 320 columns");
     const int nqb = (ldq + kUTileQ - 1) / kUTileQ;
     dim3 grid(static_cast<unsigned>(ops.n) * nqb, ops.ldc / (32 * NIW));
     // 8 x 2 waves: the 4 x 4 layout needs more than the 128 registers a
@@ -1229,7 +1235,7 @@ int exchange_default_splits(Precision prec, int n, std::size_t partial_capacity)
 }
 
 void exchange_syrk(Precision prec, const void* U, std::size_t ustride, int n, int r, int r_pos, int ldq, double* K,
-                   double* partial, hipStream_t s, std::size_t partial_capacity, int split_override) {
+                   double* partial, hipStream_t s, std::size_t partial_capacity, int split_override, int terms) {
   const std::size_t kd = static_cast<std::size_t>(r) * ldq;
   if (ustride < kd) throw std::invalid_argument("U row stride shorter than r x ldq");
   const int cap = syrk_cap(n, partial_capacity);
@@ -1253,8 +1259,9 @@ void exchange_syrk(Precision prec, const void* U, std::size_t ustride, int n, in
     if (prec == Precision::BF16X3) {
       const __bf16* wh = static_cast<const __bf16*>(U);
       const __bf16* wl = wh + static_cast<std::size_t>(n) * ustride;
-      hipLaunchKernelGGL(exchange_syrk_bf16x3_kernel<false>, dim3(grid), dim3(512), 0, s, wh, wl, ustride, n, tiles,
-                         ncb, nwork, kb, ke, chunk, sign, out32, wh, wl);
+      auto kernel = terms == 1 ? exchange_syrk_bf16x3_kernel<false, 1> : exchange_syrk_bf16x3_kernel<false, 3>;
+      hipLaunchKernelGGL(kernel, dim3(grid), dim3(512), 0, s, wh, wl, ustride, n, tiles, ncb, nwork, kb, ke, chunk, sign,
+                         out32, wh, wl);
     } else if (prec == Precision::FP64) {
       hipLaunchKernelGGL(exchange_syrk_kernel<double>, dim3(grid), dim3(512), 0, s, static_cast<const double*>(U),
                          ustride, n, tiles, ncb, nwork, kb, ke, chunk, sign, out64);
@@ -1298,7 +1305,7 @@ void exchange_cross_bf16x3(const void* X, const void* Y, std::size_t ustride, in
   const __bf16* yh = static_cast<const __bf16*>(Y);
   const __bf16* yl = yh + static_cast<std::size_t>(n) * ustride;
   float* out32 = reinterpret_cast<float*>(partial);
-  hipLaunchKernelGGL(exchange_syrk_bf16x3_kernel<true>, dim3(grid), dim3(512), 0, s, xh, xl, ustride, n, tiles, ncb, nwork,
+  hipLaunchKernelGGL((exchange_syrk_bf16x3_kernel<true, 3>), dim3(grid), dim3(512), 0, s, xh, xl, ustride, n, tiles, ncb, nwork,
                      std::size_t{0}, kd, chunk, 1.0, out32, yh, yl);
   VQC_LAUNCH("exchange_syrk_bf16x3_kernel (cross)");
   const std::size_t e = static_cast<std::size_t>(n) * n;

@@ -44,6 +44,10 @@ struct OccupiedOperands {
   float* c32 = nullptr;       // n x ldc
   void* ct_hi = nullptr;      // ldc x ldct bf16 (transposed), hi part
   void* ct_lo = nullptr;      // lo part
+  // BF16X3 products per logical one: 3 (hi hi + hi lo + lo hi, about 2^-16
+  // relative), or 1 (hi hi only, about 2^-8: the first iterations of a large
+  // mixed SCF, while the density still changes by more than 1e-2).
+  int terms = 3;
 };
 void prepare_occupied(Precision prec, OccupiedOperands& ops, hipStream_t s);
 // Elements between the rows U[mu] and U[mu + 1] (each r x ldq).  The SYRK
@@ -68,7 +72,7 @@ void occupied_u(Precision prec, const void* B, const OccupiedOperands& ops, int 
 // rule; the slices only regroup the k sum, so K changes at rounding level.
 std::size_t exchange_partial_size(int n);
 void exchange_syrk(Precision prec, const void* U, std::size_t ustride, int n, int r, int r_pos, int ldq, double* K,
-                   double* partial, hipStream_t s, std::size_t partial_capacity = 0, int splits = 0);
+                   double* partial, hipStream_t s, std::size_t partial_capacity = 0, int splits = 0, int terms = 3);
 // The split-K slices per run that the occupancy rule picks for this precision
 // and n (whole waves of resident blocks on 256 CUs), under the same cap.
 int exchange_default_splits(Precision prec, int n, std::size_t partial_capacity = 0);

@@ -40,6 +40,7 @@ struct ScfOutcome {
   unsigned reduced_iterations = 0;  // iterations run at reduced precision
   unsigned purified_iterations = 0;  // new densities from SP2 purification (the rest diagonalise)
   unsigned incremental_builds = 0;   // FP64-stage builds with K from an anchor plus a BF16X3 update
+  unsigned single_term_builds = 0;   // BF16X3-stage builds with one bf16 product per logical one
   uint64_t fock_builds = 0;
   std::vector<double> density;
   std::vector<double> forces;  // 3 natom, -dE/dR, when requested
@@ -210,6 +211,7 @@ class DfPlan {
   System orb_, aux_;
   PlanOptions opts_;
   Precision prec_ = Precision::FP64;
+  int bf16_terms_ = 3;  // BF16X3 products per logical one in build_jk (1 only inside a large mixed solve)
   int sp2_hint_ = 0;  // SP2 steps of the last purification: the next one's first batch
   int n_ = 0, naux_ = 0, ldq_ = 0, nocc_ = 0;
   std::size_t npair_ = 0;

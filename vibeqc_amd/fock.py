@@ -255,13 +255,14 @@ class FockPlan:
 
     def last_solve_stats(self) -> dict:
         """Statistics of the last solve(): iterations, iterations with
-        reduced-precision Fock builds, new densities from purification, and
-        FP64-stage builds of a mixed solve done incrementally."""
-        out = (ct.c_uint64 * 4)()
+        reduced-precision Fock builds, new densities from purification,
+        FP64-stage builds of a mixed solve done incrementally, and BF16X3
+        builds with single bf16 products."""
+        out = (ct.c_uint64 * 5)()
         with self._lock:
-            self._check(self._lib.vibeqc_fock_plan_last_solve_stats(self._handle, out, 4))
+            self._check(self._lib.vibeqc_fock_plan_last_solve_stats(self._handle, out, 5))
         return {"iterations": out[0], "reduced_iterations": out[1], "purified_densities": out[2],
-                "incremental_builds": out[3]}
+                "incremental_builds": out[3], "single_term_builds": out[4]}
 
     def one_electron(self):
         """(S, Hcore) as nbf x nbf arrays."""
