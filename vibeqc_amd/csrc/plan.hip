@@ -1262,8 +1262,8 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   // anchor is renewed once the density change has fallen kReanchor below its
   // value at the anchor (measured on (H2O)64: without this the energy wanders
   // at 1e-9 Eh, above the 1e-10 criterion, until the run of kIncrementalRun
-  // builds ends).  J is always FP64, and the finalisation below rebuilds in
-  // full FP64.
+  // builds ends).  J is always FP64, and the finalisation's last rebuild is
+  // a full FP64 build.
   constexpr int kIncrementalRun = 8;
   constexpr double kReanchor = 1e-2;
   const bool incremental = reduced && incremental_enabled(nocc);
