@@ -484,8 +484,15 @@ def _profile_run(args):
         torch.cuda.synchronize()
         _log(f"copied {args.copy_mib} MiB device-to-device 3 times")
     plan, setup, nocc = _alkane_plan() if args.only == "c20h42" else _water64_plan()
-    precisions = (args.precision,) if args.precision else PRECISIONS
-    res = _time_precisions(plan, nocc, args.steps, args.warmup, precisions)
+    if args.only == "water64_scf":
+        # The (H2O)64 SCF legs alone (mixed = the BF16X3 schedule, then FP64).
+        precisions = (args.precision,) if args.precision else ("bf16x3", "fp64")
+        res = {p: _scf_on_plan(plan, p) for p in precisions}
+        for p, r in res.items():
+            _log(f"(H2O)64 SCF {p}: {r}")
+    else:
+        precisions = (args.precision,) if args.precision else PRECISIONS
+        res = _time_precisions(plan, nocc, args.steps, args.warmup, precisions)
     plan.close()
     print(json.dumps({"workload": args.only, "setup_s": setup, "results": res}))
 
@@ -501,8 +508,9 @@ def main():
     ap.add_argument("--no-scf64", action="store_true", help="skip the (H2O)64 FP64 and mixed SCF runs")
     ap.add_argument(
         "--only",
-        choices=("c20h42", "water64"),
-        help="profiling run: time one workload only (no SCF, no CPU baseline) and print its phase times",
+        choices=("c20h42", "water64", "water64_scf"),
+        help="profiling run: time one workload only (no CPU baseline) and print its phase times; "
+        "water64_scf runs the (H2O)64 SCF legs only",
     )
     ap.add_argument("--precision", choices=PRECISIONS, help="with --only: time this precision only")
     ap.add_argument(

@@ -1161,6 +1161,14 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   DeviceArray<double> projA(sp2_usable ? static_cast<std::size_t>(n) * nocc : 1),
       projM(sp2_usable ? static_cast<std::size_t>(nocc) * nocc : 1);
   DeviceArray<rocblas_int> proj_info(1);
+  // Large systems: when the projection is not accepted, a factor from a fixed
+  // random sketch of P's range (two Cholesky-QR passes, a few library calls)
+  // is tried before the single-workgroup pivoted Cholesky, which takes about
+  // 6.5 ms at n = 1536, r = 320.
+  const bool sketch = sp2_usable && n >= 768;
+  DeviceArray<double> omega(sketch ? static_cast<std::size_t>(n) * nocc : 1);
+  DeviceArray<int> sketch_status(3);  // two pass checks, then the combined status
+  if (sketch) fill_sketch(omega.data(), omega.size(), stream_);
   int sp2_steps = 0;
   double sp2_lo = 0.0, sp2_hi = 0.0;
   const double dnocc = static_cast<double>(nocc);
@@ -1187,6 +1195,7 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
       batch = 4;
     }
   };
+  auto proj_status_download = [&](const int* status, int* host) { copy_to_host(host, status, sizeof(int), stream_); };
   // Returns true when the density came from purification; its factor's
   // status is then checked by the caller (purified_ok) before use.
   auto new_density = [&](const double* Fm, double* Dout, bool canonical) {
@@ -1199,7 +1208,17 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
         projected_factor(blas_, V.data(), n, nocc, G.data(), projA.data(), projM.data(), proj_info.data(), proj_status,
                          stream_);
       projected = have_factor;  // confirmed by the status in purified_ok
-      pivoted_cholesky(V.data(), n, nocc, G.data(), sp2_status, stream_, have_factor ? proj_status : nullptr);
+      const int* done = have_factor ? proj_status : nullptr;
+      if (sketch) {
+        int accepted = 1;
+        if (have_factor) proj_status_download(proj_status, &accepted);
+        if (accepted != 0) {
+          sketched_factor(blas_, V.data(), n, nocc, omega.data(), G.data(), projA.data(), projM.data(),
+                          proj_info.data(), sketch_status.data(), sketch_status.data() + 2, stream_);
+          done = sketch_status.data() + 2;
+        }
+      }
+      pivoted_cholesky(V.data(), n, nocc, G.data(), sp2_status, stream_, done);
       VQC_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, n, nocc, n, &one, X.data(), n,
                              G.data(), n, &zero, W.data(), n));
       purified = true;

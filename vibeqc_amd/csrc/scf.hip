@@ -417,13 +417,12 @@ __global__ __launch_bounds__(kCholThreads) void pivoted_cholesky_kernel(const do
   if (tid == 0) status[0] = 0;
 }
 
-// Projected factor check: status[0] = 0 when the Cholesky of A^T A succeeded
-// (info 0) and its diagonal spans at most kProjectedSpread (the projected
-// previous factor lost no direction of the new projector), else 1; then
-// G <- A when accepted.
+// Factor check: status[0] = 0 when the Cholesky of A^T A succeeded (info 0)
+// and its diagonal spans at most `spread` (for a projected previous factor,
+// kProjectedSpread: it lost no direction of the new projector), else 1.
 constexpr double kProjectedSpread = 1e3;
 __global__ void projected_factor_check_kernel(const rocblas_int* __restrict__ info, const double* __restrict__ R, int r,
-                                              int* __restrict__ status) {
+                                              int* __restrict__ status, double spread) {
   __shared__ double lo[256], hi[256];
   double a = INFINITY, b = 0.0;
   for (int i = threadIdx.x; i < r; i += blockDim.x) {
@@ -442,7 +441,25 @@ __global__ void projected_factor_check_kernel(const rocblas_int* __restrict__ in
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    status[0] = info[0] == 0 && lo[0] > 0.0 && isfinite(hi[0]) && hi[0] <= kProjectedSpread * lo[0] ? 0 : 1;
+    status[0] = info[0] == 0 && lo[0] > 0.0 && isfinite(hi[0]) && hi[0] <= spread * lo[0] ? 0 : 1;
+}
+
+// out[0] = 0 when both checks passed, else 1.
+__global__ void combine_status_kernel(const int* __restrict__ checks, int* __restrict__ out) {
+  if (threadIdx.x == 0) out[0] = checks[0] == 0 && checks[1] == 0 ? 0 : 1;
+}
+
+// Fixed pseudo-random sketch entries in [-1, 1): SplitMix64 of the element
+// index, so every solve of a shape uses the same sketch.
+__global__ void sketch_kernel(double* __restrict__ omega, std::size_t count) {
+  const std::size_t stride = static_cast<std::size_t>(gridDim.x) * blockDim.x;
+  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
+    uint64_t z = static_cast<uint64_t>(i) + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    omega[i] = static_cast<double>(z >> 11) * 0x1.0p-52 - 1.0;
+  }
 }
 
 __global__ void copy_if_accepted_kernel(double* __restrict__ dst, const double* __restrict__ src, std::size_t count,
@@ -590,8 +607,40 @@ void projected_factor(rocblas_handle blas, const double* P, int n, int r, double
   VQC_BLAS(rocsolver_dpotrf(blas, rocblas_fill_upper, r, M, r, info));
   VQC_BLAS(rocblas_dtrsm(blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                          rocblas_diagonal_non_unit, n, r, &one, M, r, A, n));
-  hipLaunchKernelGGL(projected_factor_check_kernel, dim3(1), dim3(256), 0, s, info, M, r, status);
+  hipLaunchKernelGGL(projected_factor_check_kernel, dim3(1), dim3(256), 0, s, info, M, r, status, kProjectedSpread);
   VQC_LAUNCH("projected_factor_check_kernel");
+  const std::size_t count = static_cast<std::size_t>(n) * r;
+  hipLaunchKernelGGL(copy_if_accepted_kernel, dim3(std::min<unsigned>(blocks_for(count), 4096u)), dim3(256), 0, s, G,
+                     A, count, status);
+  VQC_LAUNCH("copy_if_accepted_kernel");
+}
+
+void fill_sketch(double* omega, std::size_t count, hipStream_t s) {
+  hipLaunchKernelGGL(sketch_kernel, dim3(std::min<unsigned>(blocks_for(count), 4096u)), dim3(256), 0, s, omega, count);
+  VQC_LAUNCH("sketch_kernel");
+}
+
+void sketched_factor(rocblas_handle blas, const double* P, int n, int r, const double* omega, double* G, double* A,
+                     double* M, rocblas_int* info, int* checks, int* status, hipStream_t s) {
+  const double one = 1.0, zero = 0.0;
+  // A = P Omega, then two Cholesky-QR passes (A^T A = R^T R, A <- A R^-1);
+  // the second restores orthonormality lost to the first pass's conditioning.
+  VQC_BLAS(rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, n, r, n, &one, P, n, omega, n, &zero, A,
+                         n));
+  for (int pass = 0; pass < 2; ++pass) {
+    VQC_BLAS(rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, r, r, n, &one, A, n, A, n, &zero,
+                           M, r));
+    VQC_BLAS(rocsolver_dpotrf(blas, rocblas_fill_upper, r, M, r, info));
+    VQC_BLAS(rocblas_dtrsm(blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                           rocblas_diagonal_non_unit, n, r, &one, M, r, A, n));
+    // First pass: A's condition (R's diagonal spread) at most 1e6; second:
+    // A is then orthonormal to rounding, so R is the identity to rounding.
+    hipLaunchKernelGGL(projected_factor_check_kernel, dim3(1), dim3(256), 0, s, info, M, r, checks + pass,
+                       pass == 0 ? 1e6 : 1.01);
+    VQC_LAUNCH("projected_factor_check_kernel");
+  }
+  hipLaunchKernelGGL(combine_status_kernel, dim3(1), dim3(64), 0, s, checks, status);
+  VQC_LAUNCH("combine_status_kernel");
   const std::size_t count = static_cast<std::size_t>(n) * r;
   hipLaunchKernelGGL(copy_if_accepted_kernel, dim3(std::min<unsigned>(blocks_for(count), 4096u)), dim3(256), 0, s, G,
                      A, count, status);

@@ -86,5 +86,15 @@ void pivoted_cholesky(const double* P, int n, int r, double* G, int* status, hip
 // the stream.
 void projected_factor(rocblas_handle blas, const double* P, int n, int r, double* G, double* A, double* M,
                       rocblas_int* info, int* status, hipStream_t s);
+// Fixed pseudo-random entries in [-1, 1) (the same for every call).
+void fill_sketch(double* omega, std::size_t count, hipStream_t s);
+// Factor of a projector P (rank r) from a random sketch of its range:
+// A = P Omega (Omega n x r from fill_sketch), then two Cholesky-QR passes, so
+// G G^T = P (to rounding) without the previous factor.  status[0] = 0 when
+// accepted (both Cholesky factorisations succeeded, A's condition at most
+// 1e6 and orthonormal after the second pass), else 1 and G is unchanged.
+// A, M as projected_factor; `checks` holds two ints.
+void sketched_factor(rocblas_handle blas, const double* P, int n, int r, const double* omega, double* G, double* A,
+                     double* M, rocblas_int* info, int* checks, int* status, hipStream_t s);
 
 }  // namespace vqc
