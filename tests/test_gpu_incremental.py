@@ -61,3 +61,22 @@ def test_incremental_refinement_matches_fixture_and_full_builds(monkeypatch):
     assert np.abs(f1 - f0).max() < 1e-7
     assert stats64["incremental_builds"] == 0 and stats64["reduced_iterations"] == 0 and stats64["single_term_builds"] == 0
     assert abs(r64.energy - meta["energy_hartree"]) < 1e-8
+
+
+@pytest.mark.timeout(300)
+def test_sketched_factor_at_small_size(monkeypatch):
+    """The sketched projector factor (used for n >= 768 when the projection of
+    the previous factor is rejected) forced at 384 AO: same energy as the
+    default factor path and as the fixture."""
+    case, meta = ccpvdz_case("ccpvdz_water16")
+    out = {}
+    with _plan(case) as plan:
+        for mode in ("0", "1"):
+            monkeypatch.setenv("VIBEQC_SKETCH_FACTOR", mode)
+            r = plan.solve(**TIGHT)
+            out[mode] = (r.energy, plan.last_solve_stats())
+    print(f"(H2O)16 FP64: default factors {out['0']}, sketched {out['1']}")
+    assert abs(out["1"][0] - out["0"][0]) < 1e-10
+    assert abs(out["1"][0] - meta["energy_hartree"]) < 1e-8
+    assert abs(out["1"][1]["iterations"] - out["0"][1]["iterations"]) <= 1
+    assert out["1"][1]["purified_densities"] >= out["1"][1]["iterations"] - 1

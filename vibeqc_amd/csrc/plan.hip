@@ -1164,8 +1164,9 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   // Large systems: when the projection is not accepted, a factor from a fixed
   // random sketch of P's range (two Cholesky-QR passes, a few library calls)
   // is tried before the single-workgroup pivoted Cholesky, which takes about
-  // 6.5 ms at n = 1536, r = 320.
-  const bool sketch = sp2_usable && n >= 768;
+  // 6.5 ms at n = 1536, r = 320.  VIBEQC_SKETCH_FACTOR=1 uses it at any size.
+  const char* sketch_env = std::getenv("VIBEQC_SKETCH_FACTOR");
+  const bool sketch = sp2_usable && (n >= 768 || (sketch_env && sketch_env[0] == '1'));
   DeviceArray<double> omega(sketch ? static_cast<std::size_t>(n) * nocc : 1);
   DeviceArray<int> sketch_status(3);  // two pass checks, then the combined status
   if (sketch) fill_sketch(omega.data(), omega.size(), stream_);
