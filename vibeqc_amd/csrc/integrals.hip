@@ -97,12 +97,18 @@ std::vector<std::vector<int2>> pair_tasks(const System& sys, int first = 0, int 
 
 }  // namespace
 
-std::vector<double> boys_table_host() {
-  std::vector<double> t(static_cast<std::size_t>(kBoysRows) * kBoysCols);
-  for (int k = 0; k < kBoysRows; ++k)
-    for (int m = 0; m < kBoysCols; ++m)
-      t[static_cast<std::size_t>(k) * kBoysCols + m] = static_cast<double>(boys_series(m, k * static_cast<long double>(kBoysStep)));
-  return t;
+const std::vector<double>& boys_table_host() {
+  // Every plan uploads the same table; building it costs milliseconds of
+  // long-double series per plan, so it is built once (thread-safe static).
+  static const std::vector<double> table = [] {
+    std::vector<double> t(static_cast<std::size_t>(kBoysRows) * kBoysCols);
+    for (int k = 0; k < kBoysRows; ++k)
+      for (int m = 0; m < kBoysCols; ++m)
+        t[static_cast<std::size_t>(k) * kBoysCols + m] =
+            static_cast<double>(boys_series(m, k * static_cast<long double>(kBoysStep)));
+    return t;
+  }();
+  return table;
 }
 
 void upload_basis(const System& sys, DeviceBasis& out, bool auxiliary_offsets, hipStream_t s) {

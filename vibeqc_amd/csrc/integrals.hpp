@@ -27,7 +27,8 @@ constexpr int kBoysCols = 20;
 constexpr double kBoysStep = 0.1;
 constexpr double kBoysTmax = 36.0;
 constexpr int kBoysRows = 361;
-std::vector<double> boys_table_host();
+// The table on the host, computed once per process (long-double series).
+const std::vector<double>& boys_table_host();
 
 // A basis uploaded for integral evaluation.
 struct DeviceBasis {
