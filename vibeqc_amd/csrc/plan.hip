@@ -464,7 +464,7 @@ bool DfPlan::incremental_enabled(int r) const {
     if (v[0] == '0') return false;
     if (v[0] == '1') return true;
   }
-  return static_cast<double>(n_) * r * ldq_ >= 2e8;
+  return static_cast<double>(n_) * r * ldq_ >= 3e7;
 }
 
 namespace {

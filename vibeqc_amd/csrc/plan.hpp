@@ -171,7 +171,7 @@ class DfPlan {
   // Whether a BF16X3 solve with r occupied columns runs the large-system
   // schedule: single-term BF16 builds while the density change is above 1e-2
   // and incremental FP64-stage builds (VIBEQC_INCREMENTAL_K = 0 never, 1
-  // always, else for n r ldq >= 2e8, where the FP64 build dominates the solve).
+  // always, else for n r ldq >= 3e7: clusters of about 13 waters and up).
   bool incremental_enabled(int r) const;
   void upload_factor(const std::vector<double>& occupied, int nocc);
   int factorise_density(const double* D);
