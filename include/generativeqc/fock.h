@@ -235,7 +235,7 @@ GENERATIVEQC_API generativeqc_status vibeqc_fock_plan_streaming(const generative
  * FP64-stage builds of a BF16X3 solve done incrementally (exchange from the
  * last full FP64 build plus a BF16X3 update of the occupied factor's change),
  * [4] BF16X3-stage builds with one bf16 product per logical one (the first
- * iterations of a large BF16X3 solve). */
+ * iterations of a BF16X3 solve). */
 GENERATIVEQC_API generativeqc_status vibeqc_fock_plan_last_solve_stats(const generativeqc_fock_plan* plan,
                                                                        uint64_t* stats, uint64_t count);
 

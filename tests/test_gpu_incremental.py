@@ -1,9 +1,9 @@
-"""The large-system schedule of a mixed (BF16X3) SCF on the device.
+"""The cheaper stages of a mixed (BF16X3) SCF on the device.
 
-For large systems (VIBEQC_INCREMENTAL_K=1 forces it, =0 disables it) a mixed
-solve starts with single-term BF16 builds while the density change is above
-1e-2, and its FP64 refinement takes K from the last full FP64 build plus a
-BF16X3 update of the occupied factor's change.  Forced on the (H2O)16
+A mixed solve starts with single-term BF16 builds while the density change is
+above 1e-2, and its FP64 refinement takes K from the last full FP64 build plus
+a BF16X3 update of the occupied factor's change (VIBEQC_INCREMENTAL_K=0
+disables both).  Forced on the (H2O)16
 GPU4PySCF fixture: the energy stays within 1e-8 Eh and the forces within 1e-7
 Eh/bohr of the fixture, within 1e-10 Eh / 1e-7 Eh/bohr of the same solve
 without it, and both kinds of builds occur.  FP64 solves never use them."""

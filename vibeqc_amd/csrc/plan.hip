@@ -458,13 +458,10 @@ void DfPlan::build_jk(Precision p, const double* D, int r, int r_pos) {
   }
 }
 
-bool DfPlan::incremental_enabled(int r) const {
+bool DfPlan::incremental_enabled() const {
   if (streamed_ || !opts_.exchange || prec_ != Precision::BF16X3) return false;
-  if (const char* v = std::getenv("VIBEQC_INCREMENTAL_K")) {
-    if (v[0] == '0') return false;
-    if (v[0] == '1') return true;
-  }
-  return static_cast<double>(n_) * r * ldq_ >= 3e7;
+  const char* v = std::getenv("VIBEQC_INCREMENTAL_K");
+  return !(v && v[0] == '0');
 }
 
 namespace {
@@ -1273,7 +1270,7 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   double best_rms = std::numeric_limits<double>::infinity();
   int stalled = 0;
   bool in_reduced = reduced;
-  // Incremental FP64 stage (large BF16X3 solves): each full FP64 build is an
+  // Incremental FP64 stage (BF16X3 solves): each full FP64 build is an
   // anchor (its K and factor kept); the next builds take K from the anchor
   // plus a BF16X3 update whose rounding scales with the factor's change since
   // then, as long as every new factor since the anchor is the projection of
@@ -1286,8 +1283,8 @@ ScfOutcome DfPlan::solve(const ScfOptions& o, const std::vector<double>* initial
   // a full FP64 build.
   constexpr int kIncrementalRun = 8;
   constexpr double kReanchor = 1e-2;
-  const bool incremental = reduced && incremental_enabled(nocc);
-  // Single-term stage (the same large BF16X3 solves): while the density
+  const bool incremental = reduced && incremental_enabled();
+  // Single-term stage (the same BF16X3 solves): while the density
   // change is above kSingleTermRms the BF16X3 builds use one bf16 product per
   // logical one (about 2^-8 relative; (H2O)64: 36 against 59 ms per build),
   // then the three-term builds continue with a fresh DIIS history.

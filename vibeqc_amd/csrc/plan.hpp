@@ -168,11 +168,10 @@ class DfPlan {
   // BF16X3 from the U of (C + Cref)/2 and C - Cref (C = the factor in c64_,
   // r columns, all positive).  Results in J_, K_; overwrites U_.
   void build_jk_incremental(int r);
-  // Whether a BF16X3 solve with r occupied columns runs the large-system
-  // schedule: single-term BF16 builds while the density change is above 1e-2
-  // and incremental FP64-stage builds (VIBEQC_INCREMENTAL_K = 0 never, 1
-  // always, else for n r ldq >= 3e7: clusters of about 13 waters and up).
-  bool incremental_enabled(int r) const;
+  // Whether a BF16X3 solve runs the mixed schedule's two cheaper stages:
+  // single-term BF16 builds while the density change is above 1e-2 and
+  // incremental FP64-stage builds (VIBEQC_INCREMENTAL_K=0 turns them off).
+  bool incremental_enabled() const;
   void upload_factor(const std::vector<double>& occupied, int nocc);
   int factorise_density(const double* D);
   void sync();
