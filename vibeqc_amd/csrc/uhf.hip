@@ -361,6 +361,15 @@ ScfOutcome DfPlan::solve_unrestricted(const ScfOptions& o, const std::vector<dou
   double best_rms = std::numeric_limits<double>::infinity();
   int stalled = 0;
   bool in_reduced = reduced;
+  // Single-term first stage, as in the restricted solve: while the density
+  // change is above kSingleTermRms the BF16X3 builds use one bf16 product per
+  // logical one, then three-term builds continue with a fresh DIIS history.
+  constexpr double kSingleTermRms = 1e-2;
+  struct TermsReset {
+    DfPlan* plan;
+    ~TermsReset() { plan->bf16_terms_ = 3; }
+  } terms_reset{this};
+  bf16_terms_ = reduced && incremental_enabled() ? 1 : 3;
   double previous = std::numeric_limits<double>::infinity();
   bool converged = false;
   unsigned it = 0;
@@ -371,6 +380,7 @@ ScfOutcome DfPlan::solve_unrestricted(const ScfOptions& o, const std::vector<dou
     build_focks(p, D, from_orbitals ? &W : nullptr, F);
     out.fock_builds += 2;
     if (in_reduced) ++out.reduced_iterations;
+    if (in_reduced && bf16_terms_ == 1) out.single_term_builds += 2;
     energy_terms(D, F);
     const double* Fuse = F.data();
     if (cap >= 2) {
@@ -416,6 +426,13 @@ ScfOutcome DfPlan::solve_unrestricted(const ScfOptions& o, const std::vector<dou
                    energy, rms, in_reduced ? 1 : 0);
     de = std::isfinite(previous) ? std::abs(energy - previous) : std::numeric_limits<double>::infinity();
     std::swap(D, Dn);
+    if (in_reduced && bf16_terms_ == 1 && it > 1 && rms < kSingleTermRms) {
+      bf16_terms_ = 3;
+      order.clear();
+      next_slot = 0;
+      previous = std::numeric_limits<double>::infinity();
+      continue;
+    }
     if (in_reduced) {
       const bool settled = it > 1 && de < std::max(e_switch, o.energy_tolerance) &&
                            rms < std::max(d_switch, o.density_tolerance);
