@@ -280,13 +280,13 @@ def _ragged_systems(count=512, seed=2):
     return ragged_water_batch(count, seed)
 
 
-def _ragged_batch(count=512):
+def _ragged_batch(count=512, precisions=("mixed", "fp64")):
     from vibeqc_amd import Calculator
 
     systems, sizes = _ragged_systems(count)
     out = {"molecules": count, "sizes_histogram": np.bincount(sizes, minlength=17)[1:].tolist()}
     energies = {}
-    for precision in ("mixed", "fp64"):
+    for precision in precisions:
         calc = Calculator(method="rhf", basis="cc-pvdz", auxiliary_basis="cc-pvdz-jkfit", precision=precision)
         t0 = time.perf_counter()
         with calc.prepare_batch(systems, warm_start=False) as batch:
@@ -303,6 +303,8 @@ def _ragged_batch(count=512):
             "mean_iterations": float(np.mean([it.iterations for it in res.items])),
         }
         _log(f"ragged batch {precision}: {out[precision]}")
+    if len(energies) < 2:
+        return out
     err = np.abs(energies["mixed"] - energies["fp64"])
     out["max_abs_energy_error_mixed_vs_fp64"] = float(np.nanmax(err))
     out["mean_abs_energy_error_mixed_vs_fp64"] = float(np.nanmean(err))
@@ -483,6 +485,11 @@ def _profile_run(args):
             dst.copy_(src)
         torch.cuda.synchronize()
         _log(f"copied {args.copy_mib} MiB device-to-device 3 times")
+    if args.only == "ragged":
+        # The first members of the ragged batch, one precision ("fp64", else mixed).
+        precision = "fp64" if args.precision == "fp64" else "mixed"
+        print(json.dumps({"workload": "ragged", "results": _ragged_batch(args.molecules, (precision,))}))
+        return
     plan, setup, nocc = _alkane_plan() if args.only == "c20h42" else _water64_plan()
     if args.only == "water64_scf":
         # The (H2O)64 SCF legs alone (mixed = the BF16X3 schedule, then FP64).
@@ -508,10 +515,11 @@ def main():
     ap.add_argument("--no-scf64", action="store_true", help="skip the (H2O)64 FP64 and mixed SCF runs")
     ap.add_argument(
         "--only",
-        choices=("c20h42", "water64", "water64_scf"),
+        choices=("c20h42", "water64", "water64_scf", "ragged"),
         help="profiling run: time one workload only (no CPU baseline) and print its phase times; "
-        "water64_scf runs the (H2O)64 SCF legs only",
+        "water64_scf runs the (H2O)64 SCF legs only, ragged the first --molecules members of the ragged batch",
     )
+    ap.add_argument("--molecules", type=int, default=64, help="with --only ragged: batch size")
     ap.add_argument("--precision", choices=PRECISIONS, help="with --only: time this precision only")
     ap.add_argument(
         "--copy-mib",
